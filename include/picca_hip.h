@@ -1,0 +1,168 @@
+/*
+ * picca_hip.h -- C-ABI of the MI355X (gfx950) engine for picca's 3D Lya correlation path.
+ *
+ * Plain C types only: POD structs, raw host pointers and sizes.  Every function returns
+ * PICCA_OK (0) or a negative PICCA_E_* code; picca_last_error() gives the message of the
+ * last failure on the calling thread.  One context owns one device's buffers and one
+ * HIP stream; calls on one context must not overlap.
+ *
+ * Reference interfaces replaced (paths relative to the picca repository):
+ *   picca_xi_*         compute_xi            py/picca/cf.py:138-247, batched over healpix
+ *                      rows; its per-pair kernel compute_xi_forest_pairs_fast
+ *                      py/picca/cf.py:250-387 runs on the GPU for every listed pair
+ *   picca_dmat_*       compute_dmat          py/picca/cf.py:390-517 and its per-pair
+ *                      kernel compute_dmat_forest_pairs_fast py/picca/cf.py:520-887
+ *   picca_cf_params    the module globals    py/picca/cf.py:28-79 (None -> has_* = 0)
+ */
+#ifndef PICCA_HIP_H
+#define PICCA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PICCA_HIP_ABI_VERSION 1
+
+#define PICCA_OK 0
+#define PICCA_E_VALUE (-1)   /* invalid argument: picca raises ValueError          */
+#define PICCA_E_RUNTIME (-2) /* unsupported / inconsistent state: RuntimeError     */
+#define PICCA_E_INDEX (-3)   /* negative dmat data bin: IndexError (cf.py:855-856) */
+#define PICCA_E_HIP (-4)     /* HIP runtime failure                                */
+#define PICCA_E_NOMEM (-5)   /* device or host allocation failed                   */
+
+/* pair flag bits (picca_xi_set_pairs / picca_dmat_run: pair_flags) */
+#define PICCA_PAIR_SAME_HALF_PLATE 1u
+
+/*
+ * Configuration of one correlation run: the cf module globals (py/picca/cf.py:28-79)
+ * as set by picca_cf.py (py/picca/bin/picca_cf.py:343-368) and picca_dmat.py
+ * (py/picca/bin/picca_dmat.py:353-394).  A global that is None in picca has its has_*
+ * flag at 0.  edge_tol is this engine's edge-pair report threshold (Mpc/h).
+ */
+typedef struct picca_cf_params {
+    int32_t num_bins_r_par;
+    int32_t num_bins_r_trans;
+    int32_t num_model_bins_r_par;
+    int32_t num_model_bins_r_trans;
+    double r_par_min;
+    double r_par_max;
+    double r_trans_max;
+    double z_min_pairs;
+    double z_max_pairs;
+    double zerr_cut_deg;
+    double zerr_cut_kms;
+    double z_ref;
+    double alpha;
+    double alpha2;
+    double edge_tol;
+    int32_t has_z_min_pairs;
+    int32_t has_z_max_pairs;
+    int32_t has_zerr_cut;
+    int32_t x_correlation;
+    int32_t rmu_binning;
+    int32_t ang_correlation;
+    int32_t remove_same_half_plate_close_pairs;
+    int32_t redshift_evolution;
+} picca_cf_params;
+
+/* Counters of one xi execution (no counterpart in picca, which reports none). */
+typedef struct picca_xi_report {
+    int64_t accepted_pairs;  /* sum of num_pairs over all rows and bins             */
+    int64_t edge_pairs;      /* pairs within edge_tol of a bin edge or range limit */
+    int64_t overflow_pairs;  /* bin index >= np*nt (picca writes out of bounds)    */
+    int64_t visited_pairs;   /* pixel pairs the kernel evaluated after pruning     */
+    int64_t wave_steps;      /* 64-lane pair evaluations issued                     */
+    double kernel_ms;        /* device time of the pair kernel (HIP events)         */
+} picca_xi_report;
+
+typedef struct picca_ctx picca_ctx;
+
+/* version / device ---------------------------------------------------------- */
+int picca_abi_version(void);
+const char *picca_last_error(void);
+int picca_device_count(int32_t *count);
+/* name (NUL-terminated, truncated to name_len), compute units, LDS bytes per workgroup */
+int picca_device_info(int32_t device, char *name, int32_t name_len, int32_t *num_cus,
+                      int64_t *lds_per_block, int64_t *global_mem);
+
+/* context ------------------------------------------------------------------- */
+int picca_create(int32_t device, picca_ctx **out);
+int picca_destroy(picca_ctx *ctx);
+
+/*
+ * Forest store: num_forests forests, forest k owns pixels [offsets[k], offsets[k+1]).
+ * Arrays are the Delta attributes after read_deltas (py/picca/io.py:492-510):
+ * z, r_comov, dist_m, weights, delta (FP64, one value per pixel), z_qso (per forest).
+ * log_lambda and order are needed by the distortion matrix only (may be NULL).
+ * The store is copied to device memory and stays resident until replaced.
+ */
+int picca_set_forests(picca_ctx *ctx, int64_t num_forests, const int64_t *offsets,
+                      const double *z, const double *r_comov, const double *dist_m,
+                      const double *weights, const double *delta, const double *log_lambda,
+                      const double *z_qso, const int32_t *order);
+
+/*
+ * xi pair list (the (delta1, delta2) loops of compute_xi): pair k correlates forest
+ * pair_a[k] (delta1) with pair_b[k] (delta2) at angle pair_ang[k] (radians, as
+ * QSO.get_angle_between, py/picca/data.py:106-162) and accumulates into output row
+ * pair_row[k] (0 <= row < num_rows; picca_cf uses one row per delta1 healpix).  Pairs
+ * of one row must be contiguous.  cos(ang/2) and sin(ang/2) are taken with the host
+ * C library, as the reference's compiled loop does.
+ */
+int picca_xi_set_pairs(picca_ctx *ctx, int64_t num_pairs, const int32_t *pair_a,
+                       const int32_t *pair_b, const double *pair_ang,
+                       const uint8_t *pair_flags, const int32_t *pair_row, int32_t num_rows);
+
+/* Run the pair kernel over the resident pairs; results stay on the device. */
+int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_report *report);
+
+/*
+ * Copy the un-normalised sums of the last execution to the host, each [num_rows * np*nt]:
+ * sum w1 w2, sum w1 d1 w2 d2, sum r_par w1 w2, sum r_trans w1 w2, sum z w1 w2, pair
+ * counts.  Dividing by the weights where they are > 0 gives compute_xi's outputs
+ * (py/picca/cf.py:242-246).
+ */
+int picca_xi_download(picca_ctx *ctx, double *weights, double *xi, double *r_par,
+                      double *r_trans, double *z, int64_t *num_pairs);
+
+/*
+ * Distortion matrix (compute_dmat, py/picca/cf.py:390-517) over an explicit list of
+ * kept forest pairs (the rejection draws of py/picca/cf.py:444 are made by the caller,
+ * in reference order).  Needs a forest store with log_lambda and order.
+ * Outputs are accumulated into the caller's arrays:
+ *   weights_dmat[np*nt], dmat[np*nt * nm] (nm = model np*nt), r_par_eff[nm],
+ *   r_trans_eff[nm], z_eff[nm], weight_eff[nm].
+ * stats (may be NULL) receives: [0] in-range pairs, [1] selected pairs,
+ * [2] in-range pairs beyond the count-pass size (see py/picca/cf.py:619,702-703).
+ */
+int picca_dmat_run(picca_ctx *ctx, const picca_cf_params *params, int64_t num_pairs,
+                   const int32_t *pair_a, const int32_t *pair_b, const double *pair_ang,
+                   const uint8_t *pair_flags, double *weights_dmat, double *dmat,
+                   double *r_par_eff, double *r_trans_eff, double *z_eff,
+                   double *weight_eff, int64_t *stats);
+
+/*
+ * Neighbour candidates for fill_neighs (py/picca/cf.py:82-135), host code.  Replaces the
+ * healpy.query_disc(inclusive=True) superset (py/picca/cf.py:93-98): returns, per source,
+ * the targets (ascending index) with x2*x1 + y2*y1 + z2*z1 >= cos_min or inside the
+ * 2-arcsec box of the small-angle formula, excluding equal ids and, in auto mode, targets
+ * with ra >= the source's ra.  The caller applies the exact angle test.  Two passes:
+ * offsets == NULL fills counts[n_src]; otherwise the candidates are written at
+ * [offsets[s], offsets[s] + counts[s]) of out_target / out_cos.
+ */
+int picca_neighbour_candidates(int64_t n_src, const double *sx, const double *sy,
+                               const double *sz, const double *sra, const double *sdec,
+                               const int64_t *sid, int64_t n_tgt, const double *tx,
+                               const double *ty, const double *tz, const double *tra,
+                               const double *tdec, const int64_t *tid, double cos_min,
+                               double chord_max, double small_angle, int32_t auto_mode,
+                               int64_t *counts, const int64_t *offsets, int64_t *out_target,
+                               double *out_cos);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PICCA_HIP_H */

@@ -1,0 +1,308 @@
+"""CPU oracle of the correlation path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests, ``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py`` use this
+module, and only as the checker; the engine never imports it.
+
+* ``OracleParams`` and the ctypes wrappers call ``liboracle_cf.so`` (``cf_oracle.c``), the
+  scalar FP64 restatement of ``compute_xi_forest_pairs_fast`` (``py/picca/cf.py:250-387``)
+  and ``compute_dmat_forest_pairs_fast`` (``py/picca/cf.py:520-887``).
+* ``angle`` restates the scalar branch of ``QSO.get_angle_between``
+  (``py/picca/data.py:143-161``), evaluated per pair as ``compute_xi`` does
+  (``py/picca/cf.py:170``).
+* ``brute_force_neighbours`` restates ``fill_neighs`` (``py/picca/cf.py:82-135``) without a
+  disc query: it tests every forest of the target catalogue, which is the set the
+  reference's superset query reduces to after its exact angle test.
+* ``compute_xi`` / ``compute_dmat`` restate the per-healpix loops of
+  ``py/picca/cf.py:138-247`` and ``:390-517`` over ``Delta``-like objects, calling the C
+  kernel once per pair, in the reference's order.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "liboracle_cf.so")
+SMALL_ANGLE_CUT_OFF = 2.0 / 3600.0 * np.pi / 180.0
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+    return LIBRARY
+
+
+class OracleParams(ctypes.Structure):
+    _fields_ = [
+        ("num_bins_r_par", ctypes.c_int32),
+        ("num_bins_r_trans", ctypes.c_int32),
+        ("num_model_bins_r_par", ctypes.c_int32),
+        ("num_model_bins_r_trans", ctypes.c_int32),
+        ("r_par_min", ctypes.c_double),
+        ("r_par_max", ctypes.c_double),
+        ("r_trans_max", ctypes.c_double),
+        ("z_min_pairs", ctypes.c_double),
+        ("z_max_pairs", ctypes.c_double),
+        ("zerr_cut_deg", ctypes.c_double),
+        ("zerr_cut_kms", ctypes.c_double),
+        ("z_ref", ctypes.c_double),
+        ("alpha", ctypes.c_double),
+        ("alpha2", ctypes.c_double),
+        ("edge_tol", ctypes.c_double),
+        ("has_z_min_pairs", ctypes.c_int32),
+        ("has_z_max_pairs", ctypes.c_int32),
+        ("has_zerr_cut", ctypes.c_int32),
+        ("x_correlation", ctypes.c_int32),
+        ("rmu_binning", ctypes.c_int32),
+        ("ang_correlation", ctypes.c_int32),
+        ("remove_same_half_plate_close_pairs", ctypes.c_int32),
+        ("redshift_evolution", ctypes.c_int32),
+    ]
+
+
+def make_params(num_bins_r_par, num_bins_r_trans, r_par_min, r_par_max, r_trans_max,
+                z_min_pairs=None, z_max_pairs=None, zerr_cut_deg=None, zerr_cut_kms=None,
+                num_model_bins_r_par=0, num_model_bins_r_trans=0, z_ref=2.25, alpha=2.9,
+                alpha2=2.9, x_correlation=False, rmu_binning=False, ang_correlation=False,
+                remove_same_half_plate_close_pairs=False, redshift_evolution=True,
+                edge_tol=1e-9):
+    p = OracleParams()
+    p.num_bins_r_par = num_bins_r_par
+    p.num_bins_r_trans = num_bins_r_trans
+    p.num_model_bins_r_par = num_model_bins_r_par
+    p.num_model_bins_r_trans = num_model_bins_r_trans
+    p.r_par_min = r_par_min
+    p.r_par_max = r_par_max
+    p.r_trans_max = r_trans_max
+    p.has_z_min_pairs = int(z_min_pairs is not None)
+    p.z_min_pairs = z_min_pairs if z_min_pairs is not None else float("nan")
+    p.has_z_max_pairs = int(z_max_pairs is not None)
+    p.z_max_pairs = z_max_pairs if z_max_pairs is not None else float("nan")
+    p.has_zerr_cut = int(zerr_cut_deg is not None)
+    p.zerr_cut_deg = zerr_cut_deg if zerr_cut_deg is not None else float("nan")
+    p.zerr_cut_kms = zerr_cut_kms if zerr_cut_kms is not None else float("nan")
+    p.z_ref = z_ref
+    p.alpha = alpha
+    p.alpha2 = alpha2
+    p.edge_tol = edge_tol
+    p.x_correlation = int(x_correlation)
+    p.rmu_binning = int(rmu_binning)
+    p.ang_correlation = int(ang_correlation)
+    p.remove_same_half_plate_close_pairs = int(remove_same_half_plate_close_pairs)
+    p.redshift_evolution = int(redshift_evolution)
+    return p
+
+
+def params_from_module(cf, edge_tol=1e-9):
+    """OracleParams from an object carrying picca.cf-style globals."""
+    return make_params(
+        cf.num_bins_r_par, cf.num_bins_r_trans, cf.r_par_min, cf.r_par_max, cf.r_trans_max,
+        cf.z_min_pairs, cf.z_max_pairs, cf.zerr_cut_deg, cf.zerr_cut_kms,
+        cf.num_model_bins_r_par or 0, cf.num_model_bins_r_trans or 0,
+        cf.z_ref if cf.z_ref is not None else float("nan"),
+        cf.alpha if cf.alpha is not None else float("nan"),
+        cf.alpha2 if cf.alpha2 is not None else float("nan"),
+        bool(cf.x_correlation), bool(cf.rmu_binning), bool(cf.ang_correlation),
+        bool(cf.remove_same_half_plate_close_pairs),
+        bool(getattr(cf, "redshift_evolution_in_distortion_matrix", True)), edge_tol)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY):
+            build()
+        _lib = ctypes.CDLL(LIBRARY)
+        P = ctypes.c_void_p
+        _lib.oracle_xi_pair.argtypes = ([ctypes.POINTER(OracleParams)] + [P] * 5
+                                        + [ctypes.c_int64, ctypes.c_double] + [P] * 5
+                                        + [ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_int32] + [P] * 7)
+        _lib.oracle_xi_pair.restype = None
+        _lib.oracle_xi_rows.argtypes = ([ctypes.POINTER(OracleParams)] + [P] * 7
+                                        + [ctypes.c_int64] + [P] * 5 + [ctypes.c_int32]
+                                        + [P] * 7 + [ctypes.c_int32])
+        _lib.oracle_xi_rows.restype = None
+        _lib.oracle_dmat_pair.argtypes = ([ctypes.POINTER(OracleParams)] + [P] * 10
+                                          + [ctypes.c_int64, ctypes.c_int64, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_double] + [P] * 6
+                                          + [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, P])
+        _lib.oracle_dmat_pair.restype = ctypes.c_int
+        _lib.oracle_dmat_rows.argtypes = ([ctypes.POINTER(OracleParams)] + [P] * 8
+                                          + [ctypes.c_int64] + [P] * 4 + [P] * 7)
+        _lib.oracle_dmat_rows.restype = ctypes.c_int
+    return _lib
+
+
+def _p(a):
+    return None if a is None else np.ascontiguousarray(a).ctypes.data
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def xi_pair(params, z1, r1, d1, w1, delta1, zq1, z2, r2, d2, w2, delta2, zq2, ang,
+            same_half_plate, out, stats):
+    """One forest pair; ``out`` = (we, xi, rp, rt, z, nb) arrays updated in place."""
+    arrays = [_f(x) for x in (z1, r1, d1, w1, delta1, z2, r2, d2, w2, delta2)]
+    lib().oracle_xi_pair(ctypes.byref(params), *[_p(a) for a in arrays[:5]], len(arrays[0]),
+                         float(zq1), *[_p(a) for a in arrays[5:]], len(arrays[5]), float(zq2),
+                         float(ang), int(bool(same_half_plate)), *[_p(a) for a in out],
+                         _p(stats))
+
+
+def xi_rows(params, store, pair_a, pair_b, pair_ang, pair_flags, pair_row, num_rows,
+            num_threads=1):
+    """Batched restatement of compute_xi over rows (pairs of a row visited in input order).
+
+    Returns (we, xi, rp, rt, z, nb) un-normalised arrays [num_rows, nb] and stats[3].
+    """
+    nb = params.num_bins_r_par * params.num_bins_r_trans
+    pair_row = np.asarray(pair_row)
+    order = np.argsort(pair_row, kind="stable")
+    a = np.ascontiguousarray(np.asarray(pair_a)[order], dtype=np.int32)
+    b = np.ascontiguousarray(np.asarray(pair_b)[order], dtype=np.int32)
+    ang = np.ascontiguousarray(np.asarray(pair_ang)[order], dtype=np.float64)
+    shp = np.ascontiguousarray(np.asarray(pair_flags)[order] & 1, dtype=np.uint8)
+    counts = np.bincount(pair_row[order], minlength=num_rows)
+    row_start = np.zeros(num_rows + 1, dtype=np.int64)
+    np.cumsum(counts, out=row_start[1:])
+    out = [np.zeros((num_rows, nb)) for _ in range(5)] + [np.zeros((num_rows, nb), dtype=np.int64)]
+    stats = np.zeros(3, dtype=np.int64)
+    lib().oracle_xi_rows(ctypes.byref(params), _p(store.offsets), _p(store.z),
+                         _p(store.r_comov), _p(store.dist_m), _p(store.weights),
+                         _p(store.delta), _p(store.z_qso), len(a), _p(a), _p(b), _p(ang),
+                         _p(shp), _p(row_start), int(num_rows), *[_p(x) for x in out],
+                         _p(stats), int(num_threads))
+    return out, stats
+
+
+def dmat_pair(params, ll1, ll2, r1, r2, d1, d2, z1, z2, w1, w2, zq1, zq2, ang, out,
+              same_half_plate, order1, order2, stats):
+    arrays = [_f(x) for x in (ll1, ll2, r1, r2, d1, d2, z1, z2, w1, w2)]
+    status = lib().oracle_dmat_pair(ctypes.byref(params), *[_p(a) for a in arrays],
+                                    len(arrays[0]), len(arrays[1]), float(zq1), float(zq2),
+                                    float(ang), *[_p(x) for x in out],
+                                    int(bool(same_half_plate)), int(order1), int(order2),
+                                    _p(stats))
+    if status == -1:
+        raise IndexError("negative bin index")
+    if status != 0:
+        raise MemoryError("oracle_dmat_pair failed")
+
+
+# ---------------------------------------------------------------------------
+# pure-Python restatements of the host loops
+# ---------------------------------------------------------------------------
+
+def angle(d1, d2):
+    """Scalar branch of QSO.get_angle_between (py/picca/data.py:143-161)."""
+    cos = d2.x_cart * d1.x_cart + d2.y_cart * d1.y_cart + d2.z_cart * d1.z_cart
+    if cos >= 1.0:
+        cos = 1.0
+    elif cos <= -1.0:
+        cos = -1.0
+    ang = np.arccos(cos)
+    if (np.absolute(d2.ra - d1.ra) < SMALL_ANGLE_CUT_OFF) & (
+            np.absolute(d2.dec - d1.dec) < SMALL_ANGLE_CUT_OFF):
+        ang = np.sqrt((d2.dec - d1.dec) ** 2 + (d1.cos_dec * (d2.ra - d1.ra)) ** 2)
+    return ang
+
+
+def brute_force_neighbours(data, data2, healpixs, ang_max):
+    """{id(delta1): [delta2, ...]} for every delta1 of ``healpixs`` (py/picca/cf.py:82-135)."""
+    targets_dict = data2 if data2 is not None else data
+    targets = [d for hp in sorted(targets_dict) for d in targets_dict[hp]]
+    tx = np.array([d.x_cart for d in targets])
+    ty = np.array([d.y_cart for d in targets])
+    tz = np.array([d.z_cart for d in targets])
+    tra = np.array([d.ra for d in targets])
+    tdec = np.array([d.dec for d in targets])
+    tid = np.array([d.thingid for d in targets])
+    out = {}
+    for hp in healpixs:
+        for d1 in data[hp]:
+            # the list branch of get_angle_between (py/picca/data.py:118-141) over all targets
+            cos = tx * d1.x_cart + ty * d1.y_cart + tz * d1.z_cart
+            cos[cos >= 1.0] = 1.0
+            cos[cos <= -1.0] = -1.0
+            angs = np.arccos(cos)
+            close = ((np.absolute(tra - d1.ra) < SMALL_ANGLE_CUT_OFF)
+                     & (np.absolute(tdec - d1.dec) < SMALL_ANGLE_CUT_OFF))
+            if close.any():
+                angs[close] = np.sqrt((tdec[close] - d1.dec) ** 2
+                                      + (d1.cos_dec * (tra[close] - d1.ra)) ** 2)
+            sel = (tid != d1.thingid) & (angs < ang_max)
+            neigh = [targets[k] for k in np.nonzero(sel)[0]]
+            if data2 is None:
+                neigh = [d for d in neigh if d1.ra > d.ra]
+            out[id(d1)] = neigh
+    return out
+
+
+def same_half_plate(d1, d2, remove):
+    if not remove:
+        return False
+    return (d1.plate == d2.plate) and ((d1.fiberid <= 500 and d2.fiberid <= 500)
+                                       or (d1.fiberid > 500 and d2.fiberid > 500))
+
+
+def compute_xi(params, data, healpixs, neighbours, ang_correlation=False):
+    """compute_xi (py/picca/cf.py:138-247): returns normalised outputs and stats."""
+    nb = params.num_bins_r_par * params.num_bins_r_trans
+    out = [np.zeros(nb) for _ in range(5)] + [np.zeros(nb, dtype=np.int64)]
+    stats = np.zeros(3, dtype=np.int64)
+    for hp in healpixs:
+        for d1 in data[hp]:
+            for d2 in neighbours[id(d1)]:
+                ang = angle(d1, d2)
+                shp = same_half_plate(d1, d2, params.remove_same_half_plate_close_pairs)
+                if ang_correlation:
+                    xi_pair(params, d1.z, 10.0 ** d1.log_lambda, 10.0 ** d1.log_lambda,
+                            d1.weights, d1.delta, d1.z_qso, d2.z, 10.0 ** d2.log_lambda,
+                            10.0 ** d2.log_lambda, d2.weights, d2.delta, d2.z_qso, ang, shp,
+                            out, stats)
+                else:
+                    xi_pair(params, d1.z, d1.r_comov, d1.dist_m, d1.weights, d1.delta,
+                            d1.z_qso, d2.z, d2.r_comov, d2.dist_m, d2.weights, d2.delta,
+                            d2.z_qso, ang, shp, out, stats)
+    we, xi, rp, rt, z, npairs = out
+    w = we > 0
+    for arr in (xi, rp, rt, z):
+        arr[w] /= we[w]
+    return (we, xi, rp, rt, z, npairs), stats
+
+
+def compute_dmat(params, data, healpixs, neighbours, reject):
+    """compute_dmat (py/picca/cf.py:390-517) with the legacy global numpy stream."""
+    nb = params.num_bins_r_par * params.num_bins_r_trans
+    nm = params.num_model_bins_r_par * params.num_model_bins_r_trans
+    weights_dmat = np.zeros(nb)
+    dmat = np.zeros(nb * nm)
+    eff = [np.zeros(nm) for _ in range(4)]
+    stats = np.zeros(3, dtype=np.int64)
+    num_pairs = 0
+    num_pairs_used = 0
+    for hp in healpixs:
+        for d1 in data[hp]:
+            if d1.order is None:
+                raise RuntimeError("order is not defined for the deltas")
+            neigh = neighbours[id(d1)]
+            w = np.random.rand(len(neigh)) > reject
+            num_pairs += len(neigh)
+            num_pairs_used += w.sum()
+            for d2, keep in zip(neigh, w):
+                if not keep:
+                    continue
+                shp = same_half_plate(d1, d2, params.remove_same_half_plate_close_pairs)
+                ang = angle(d1, d2)
+                dmat_pair(params, d1.log_lambda, d2.log_lambda, d1.r_comov, d2.r_comov,
+                          d1.dist_m, d2.dist_m, d1.z, d2.z, d1.weights, d2.weights,
+                          d1.z_qso, d2.z_qso, ang, [weights_dmat, dmat] + eff, shp,
+                          d1.order, d2.order, stats)
+    return (weights_dmat, dmat.reshape(nb, nm), eff[0], eff[1], eff[2], eff[3], num_pairs,
+            int(num_pairs_used)), stats

@@ -1,0 +1,461 @@
+// C-ABI of the engine (declared in include/picca_hip.h): device context, resident forest
+// store and pair lists, launches, result download and error reporting.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/picca_hip.h"
+#include "engine.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define PICCA_HIP_CHECK(expr)                                                              \
+    do {                                                                                   \
+        hipError_t err_ = (expr);                                                          \
+        if (err_ != hipSuccess)                                                            \
+            return fail(PICCA_E_HIP, std::string(#expr) + ": " + hipGetErrorString(err_)); \
+    } while (0)
+
+template <typename T>
+struct DeviceBuffer {
+    T *ptr = nullptr;
+    size_t count = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    hipError_t reserve(size_t n) {
+        if (n <= count && ptr) return hipSuccess;
+        release();
+        size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        hipError_t err = hipMalloc((void **)&ptr, bytes);
+        if (err != hipSuccess) {
+            ptr = nullptr;
+            return err;
+        }
+        count = n;
+        return hipSuccess;
+    }
+    hipError_t upload(const T *host, size_t n, hipStream_t stream) {
+        hipError_t err = reserve(n);
+        if (err != hipSuccess || n == 0) return err;
+        return hipMemcpyAsync(ptr, host, n * sizeof(T), hipMemcpyHostToDevice, stream);
+    }
+};
+
+// Runs body(begin, end) over [0, n) on up to hardware_concurrency threads.
+template <typename F>
+void parallel_for(int64_t n, F body) {
+    unsigned hw = std::thread::hardware_concurrency();
+    int64_t nthreads = std::max<int64_t>(1, std::min<int64_t>(hw ? hw : 1, 16));
+    if (n < 65536) nthreads = 1;
+    if (nthreads == 1) {
+        body(0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    int64_t step = (n + nthreads - 1) / nthreads;
+    for (int64_t t = 0; t < nthreads; ++t) {
+        int64_t lo = t * step, hi = std::min(n, lo + step);
+        if (lo >= hi) break;
+        pool.emplace_back([=] { body(lo, hi); });
+    }
+    for (auto &th : pool) th.join();
+}
+
+}  // namespace
+
+struct picca_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    int num_cus = 0;
+    int64_t lds_limit = 0;
+
+    // forest store
+    int64_t num_forests = 0;
+    int64_t num_pixels = 0;
+    bool has_log_lambda = false;
+    bool has_order = false;
+    std::vector<int64_t> h_offsets;
+    DeviceBuffer<double> z, r, d, w, dw, ll, zq;
+    DeviceBuffer<int64_t> off;
+    DeviceBuffer<int32_t> order;
+    DeviceBuffer<uint8_t> mono;
+
+    // xi pairs
+    int64_t num_pairs = 0;
+    int32_t num_rows = 0;
+    DeviceBuffer<int32_t> pa, pb;
+    DeviceBuffer<double> pang, pcs;
+    DeviceBuffer<uint8_t> pflags;
+    std::vector<int64_t> h_group_begin;
+    std::vector<int32_t> h_group_row;
+    DeviceBuffer<int64_t> gbegin;
+    DeviceBuffer<int32_t> grow;
+
+    // xi results
+    int64_t out_nb = 0;
+    int32_t out_rows = 0;
+    bool have_result = false;
+    DeviceBuffer<double> o_we, o_xi, o_rp, o_rt, o_z;
+    DeviceBuffer<unsigned long long> o_nb, queue, stats;
+};
+
+namespace {
+
+bool forest_monotone(const double *x, int64_t n) {
+    for (int64_t k = 1; k < n; ++k)
+        if (!(x[k] >= x[k - 1])) return false;
+    return true;
+}
+
+int build_xi_const(const picca_cf_params *p, picca::XiConst &C, int &mode) {
+    if (!p) return fail(PICCA_E_VALUE, "params is NULL");
+    if (p->num_bins_r_par <= 0 || p->num_bins_r_trans <= 0)
+        return fail(PICCA_E_VALUE, "num_bins_r_par and num_bins_r_trans must be positive");
+    if (!(p->r_par_max > p->r_par_min))
+        return fail(PICCA_E_VALUE, "r_par_max must be larger than r_par_min");
+    if (!(p->r_trans_max > 0))
+        return fail(PICCA_E_VALUE, "r_trans_max must be positive");
+    if (!(p->edge_tol >= 0)) return fail(PICCA_E_VALUE, "edge_tol must be >= 0");
+    if (p->has_zerr_cut && !(p->zerr_cut_kms == p->zerr_cut_kms))
+        return fail(PICCA_E_VALUE, "zerr_cut_kms must be given with zerr_cut_deg");
+    std::memset(&C, 0, sizeof(C));
+    C.np = p->num_bins_r_par;
+    C.nt = p->num_bins_r_trans;
+    C.nb = C.np * C.nt;
+    C.rpmin = p->r_par_min;
+    C.rpmax = p->r_par_max;
+    C.rtmax = p->r_trans_max;
+    C.span_p = p->r_par_max - p->r_par_min;
+    C.np_d = (double)C.np;
+    C.nt_d = (double)C.nt;
+    C.nb_d = (double)C.nb;
+    C.inv_p = C.np_d / C.span_p;
+    C.inv_t = C.nt_d / C.rtmax;
+    C.width_p = C.span_p / C.np;
+    C.width_t = C.rtmax / C.nt;
+    C.edge_tol = p->edge_tol;
+    // |t_fast - t_exact| <= 4 ulp(t) << 1e-12 for t < 2^20; the margin also absorbs the
+    // rounding of frac * width in the edge test.
+    C.fast_p = p->edge_tol / C.width_p * (1.0 + 1e-6) + 1e-12;
+    C.fast_t = p->edge_tol / C.width_t * (1.0 + 1e-6) + 1e-12;
+    C.zmin = p->z_min_pairs;
+    C.zmax = p->z_max_pairs;
+    C.zerr_ang = p->zerr_cut_deg * M_PI / 180.0;
+    C.zerr_kms = p->zerr_cut_kms;
+    int flags = 0;
+    if (p->has_z_min_pairs) flags |= picca::XI_HAS_ZMIN;
+    if (p->has_z_max_pairs) flags |= picca::XI_HAS_ZMAX;
+    if (p->has_zerr_cut) flags |= picca::XI_HAS_ZERR;
+    if (p->remove_same_half_plate_close_pairs) flags |= picca::XI_REMOVE_SHP;
+    mode = p->ang_correlation ? picca::MODE_ANGULAR
+                              : (p->rmu_binning ? picca::MODE_RMU : picca::MODE_STANDARD);
+    if (mode == picca::MODE_STANDARD) flags |= picca::XI_PRUNE;
+    C.flags = flags;
+    return PICCA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int picca_abi_version(void) { return PICCA_HIP_ABI_VERSION; }
+
+const char *picca_last_error(void) { return g_last_error.c_str(); }
+
+int picca_device_count(int32_t *count) {
+    if (!count) return fail(PICCA_E_VALUE, "count is NULL");
+    int n = 0;
+    hipError_t err = hipGetDeviceCount(&n);
+    if (err != hipSuccess) {
+        *count = 0;
+        return fail(PICCA_E_HIP, std::string("hipGetDeviceCount: ") + hipGetErrorString(err));
+    }
+    *count = n;
+    return PICCA_OK;
+}
+
+int picca_device_info(int32_t device, char *name, int32_t name_len, int32_t *num_cus,
+                      int64_t *lds_per_block, int64_t *global_mem) {
+    hipDeviceProp_t prop;
+    PICCA_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+    if (name && name_len > 0) {
+        std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (num_cus) *num_cus = prop.multiProcessorCount;
+    if (lds_per_block) *lds_per_block = (int64_t)prop.maxSharedMemoryPerMultiProcessor;
+    if (global_mem) *global_mem = (int64_t)prop.totalGlobalMem;
+    return PICCA_OK;
+}
+
+int picca_create(int32_t device, picca_ctx **out) {
+    if (!out) return fail(PICCA_E_VALUE, "out is NULL");
+    *out = nullptr;
+    int count = 0;
+    PICCA_HIP_CHECK(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        return fail(PICCA_E_VALUE, "device " + std::to_string(device) + " out of range");
+    PICCA_HIP_CHECK(hipSetDevice(device));
+    picca_ctx *ctx = new picca_ctx();
+    ctx->device = device;
+    hipDeviceProp_t prop;
+    hipError_t err = hipGetDeviceProperties(&prop, device);
+    if (err == hipSuccess) err = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipEventCreate(&ctx->ev_start);
+    if (err == hipSuccess) err = hipEventCreate(&ctx->ev_stop);
+    if (err != hipSuccess) {
+        picca_destroy(ctx);
+        return fail(PICCA_E_HIP, std::string("picca_create: ") + hipGetErrorString(err));
+    }
+    ctx->num_cus = prop.multiProcessorCount;
+    ctx->lds_limit = (int64_t)prop.maxSharedMemoryPerMultiProcessor;
+    *out = ctx;
+    return PICCA_OK;
+}
+
+int picca_destroy(picca_ctx *ctx) {
+    if (!ctx) return PICCA_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+    if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PICCA_OK;
+}
+
+int picca_set_forests(picca_ctx *ctx, int64_t num_forests, const int64_t *offsets,
+                      const double *z, const double *r_comov, const double *dist_m,
+                      const double *weights, const double *delta, const double *log_lambda,
+                      const double *z_qso, const int32_t *order) {
+    if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
+    if (num_forests < 0 || num_forests > INT32_MAX - 1)
+        return fail(PICCA_E_VALUE, "num_forests out of range");
+    if (!offsets || !z || !r_comov || !dist_m || !weights || !delta || !z_qso)
+        return fail(PICCA_E_VALUE, "forest arrays must not be NULL");
+    if (offsets[0] != 0) return fail(PICCA_E_VALUE, "offsets[0] must be 0");
+    for (int64_t k = 0; k < num_forests; ++k) {
+        if (offsets[k + 1] < offsets[k]) return fail(PICCA_E_VALUE, "offsets must be non-decreasing");
+        if (offsets[k + 1] - offsets[k] > INT32_MAX / 2)
+            return fail(PICCA_E_VALUE, "forest too long");
+    }
+    const int64_t npix = offsets[num_forests];
+    PICCA_HIP_CHECK(hipSetDevice(ctx->device));
+    PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+
+    std::vector<double> dw((size_t)npix);
+    parallel_for(npix, [&](int64_t lo, int64_t hi) {
+        for (int64_t k = lo; k < hi; ++k) dw[k] = delta[k] * weights[k];
+    });
+    std::vector<uint8_t> mono((size_t)std::max<int64_t>(num_forests, 1));
+    parallel_for(num_forests, [&](int64_t lo, int64_t hi) {
+        for (int64_t k = lo; k < hi; ++k) {
+            const int64_t o = offsets[k], n = offsets[k + 1] - offsets[k];
+            mono[k] = (uint8_t)((forest_monotone(r_comov + o, n) ? 1 : 0) |
+                                (forest_monotone(dist_m + o, n) ? 2 : 0));
+        }
+    });
+    ctx->have_result = false;
+    ctx->num_pairs = 0;
+    ctx->h_offsets.assign(offsets, offsets + num_forests + 1);
+    PICCA_HIP_CHECK(ctx->off.upload(offsets, (size_t)num_forests + 1, ctx->stream));
+    PICCA_HIP_CHECK(ctx->z.upload(z, (size_t)npix, ctx->stream));
+    PICCA_HIP_CHECK(ctx->r.upload(r_comov, (size_t)npix, ctx->stream));
+    PICCA_HIP_CHECK(ctx->d.upload(dist_m, (size_t)npix, ctx->stream));
+    PICCA_HIP_CHECK(ctx->w.upload(weights, (size_t)npix, ctx->stream));
+    PICCA_HIP_CHECK(ctx->dw.upload(dw.data(), (size_t)npix, ctx->stream));
+    PICCA_HIP_CHECK(ctx->zq.upload(z_qso, (size_t)num_forests, ctx->stream));
+    PICCA_HIP_CHECK(ctx->mono.upload(mono.data(), (size_t)num_forests, ctx->stream));
+    ctx->has_log_lambda = log_lambda != nullptr;
+    if (log_lambda) PICCA_HIP_CHECK(ctx->ll.upload(log_lambda, (size_t)npix, ctx->stream));
+    ctx->has_order = order != nullptr;
+    if (order) PICCA_HIP_CHECK(ctx->order.upload(order, (size_t)num_forests, ctx->stream));
+    PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->num_forests = num_forests;
+    ctx->num_pixels = npix;
+    return PICCA_OK;
+}
+
+int picca_xi_set_pairs(picca_ctx *ctx, int64_t num_pairs, const int32_t *pair_a,
+                       const int32_t *pair_b, const double *pair_ang,
+                       const uint8_t *pair_flags, const int32_t *pair_row, int32_t num_rows) {
+    if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
+    if (num_pairs < 0 || num_rows < 0) return fail(PICCA_E_VALUE, "negative size");
+    if (num_pairs > 0 && (!pair_a || !pair_b || !pair_ang || !pair_flags || !pair_row))
+        return fail(PICCA_E_VALUE, "pair arrays must not be NULL");
+    const int64_t nf = ctx->num_forests;
+    // validation: indices, rows, contiguity of rows
+    std::vector<uint8_t> row_seen((size_t)std::max(num_rows, 1), 0);
+    for (int64_t k = 0; k < num_pairs; ++k) {
+        if (pair_a[k] < 0 || pair_a[k] >= nf || pair_b[k] < 0 || pair_b[k] >= nf)
+            return fail(PICCA_E_VALUE, "pair index out of range of the forest store");
+        const int32_t row = pair_row[k];
+        if (row < 0 || row >= num_rows) return fail(PICCA_E_VALUE, "pair row out of range");
+        if (k == 0 || row != pair_row[k - 1]) {
+            if (row_seen[row]) return fail(PICCA_E_VALUE, "pairs of one row must be contiguous");
+            row_seen[row] = 1;
+        }
+        if (!(pair_ang[k] >= 0.0) || !(pair_ang[k] <= M_PI))
+            return fail(PICCA_E_VALUE, "pair angle outside [0, pi]");
+    }
+    PICCA_HIP_CHECK(hipSetDevice(ctx->device));
+    PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+
+    std::vector<double> cs((size_t)std::max<int64_t>(2 * num_pairs, 1));
+    parallel_for(num_pairs, [&](int64_t lo, int64_t hi) {
+        for (int64_t k = lo; k < hi; ++k) {
+            cs[2 * k] = std::cos(pair_ang[k] / 2);
+            cs[2 * k + 1] = std::sin(pair_ang[k] / 2);
+        }
+    });
+    // groups: <= XI_GROUP_PAIRS consecutive pairs of one row
+    ctx->h_group_begin.clear();
+    ctx->h_group_row.clear();
+    for (int64_t k = 0; k < num_pairs;) {
+        int64_t end = k + 1;
+        while (end < num_pairs && pair_row[end] == pair_row[k] && end - k < picca::XI_GROUP_PAIRS)
+            ++end;
+        ctx->h_group_begin.push_back(k);
+        ctx->h_group_row.push_back(pair_row[k]);
+        k = end;
+    }
+    ctx->h_group_begin.push_back(num_pairs);
+
+    ctx->have_result = false;
+    PICCA_HIP_CHECK(ctx->pa.upload(pair_a, (size_t)num_pairs, ctx->stream));
+    PICCA_HIP_CHECK(ctx->pb.upload(pair_b, (size_t)num_pairs, ctx->stream));
+    PICCA_HIP_CHECK(ctx->pang.upload(pair_ang, (size_t)num_pairs, ctx->stream));
+    PICCA_HIP_CHECK(ctx->pcs.upload(cs.data(), (size_t)(2 * num_pairs), ctx->stream));
+    PICCA_HIP_CHECK(ctx->pflags.upload(pair_flags, (size_t)num_pairs, ctx->stream));
+    PICCA_HIP_CHECK(ctx->gbegin.upload(ctx->h_group_begin.data(), ctx->h_group_begin.size(),
+                                       ctx->stream));
+    PICCA_HIP_CHECK(ctx->grow.upload(ctx->h_group_row.data(), ctx->h_group_row.size(),
+                                     ctx->stream));
+    PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->num_pairs = num_pairs;
+    ctx->num_rows = num_rows;
+    return PICCA_OK;
+}
+
+int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_report *report) {
+    if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
+    picca::XiConst C;
+    int mode = 0;
+    int status = build_xi_const(params, C, mode);
+    if (status != PICCA_OK) return status;
+    // LDS: histogram (5 FP64 + 1 u32 per bin) + one staging tile per wave
+    const size_t hist_bytes = (size_t)C.nb * 5 * sizeof(double) + (size_t)((C.nb + 1) / 2) * 8;
+    const size_t static_lds = 1024;
+    const int waves = picca::XI_BLOCK / 64;
+    size_t lds = 0;
+    for (int tile = 64; tile >= 8; tile /= 2) {
+        lds = hist_bytes + (size_t)waves * 5 * tile * sizeof(double);
+        C.tile_len = tile;
+        if ((int64_t)(lds + static_lds) <= ctx->lds_limit) break;
+    }
+    if ((int64_t)(lds + static_lds) > ctx->lds_limit)
+        return fail(PICCA_E_VALUE, "np*nt = " + std::to_string(C.nb) +
+                                       " bins do not fit the LDS histogram (max " +
+                                       std::to_string((ctx->lds_limit - static_lds -
+                                                       waves * 5 * 8 * 8) / 44) + ")");
+    PICCA_HIP_CHECK(hipSetDevice(ctx->device));
+    const int64_t nrows = ctx->num_rows;
+    const int64_t nout = nrows * C.nb;
+    PICCA_HIP_CHECK(ctx->o_we.reserve((size_t)nout));
+    PICCA_HIP_CHECK(ctx->o_xi.reserve((size_t)nout));
+    PICCA_HIP_CHECK(ctx->o_rp.reserve((size_t)nout));
+    PICCA_HIP_CHECK(ctx->o_rt.reserve((size_t)nout));
+    PICCA_HIP_CHECK(ctx->o_z.reserve((size_t)nout));
+    PICCA_HIP_CHECK(ctx->o_nb.reserve((size_t)nout));
+    PICCA_HIP_CHECK(ctx->queue.reserve(1));
+    PICCA_HIP_CHECK(ctx->stats.reserve(picca::XI_NSTATS));
+    PICCA_HIP_CHECK(picca::launch_zero(ctx->o_we.ptr, nout, ctx->stream));
+    PICCA_HIP_CHECK(picca::launch_zero(ctx->o_xi.ptr, nout, ctx->stream));
+    PICCA_HIP_CHECK(picca::launch_zero(ctx->o_rp.ptr, nout, ctx->stream));
+    PICCA_HIP_CHECK(picca::launch_zero(ctx->o_rt.ptr, nout, ctx->stream));
+    PICCA_HIP_CHECK(picca::launch_zero(ctx->o_z.ptr, nout, ctx->stream));
+    PICCA_HIP_CHECK(picca::launch_zero_u64(ctx->o_nb.ptr, nout, ctx->stream));
+    PICCA_HIP_CHECK(picca::launch_zero_u64(ctx->queue.ptr, 1, ctx->stream));
+    PICCA_HIP_CHECK(picca::launch_zero_u64(ctx->stats.ptr, picca::XI_NSTATS, ctx->stream));
+
+    picca::ForestView F{ctx->z.ptr, ctx->r.ptr, ctx->d.ptr, ctx->w.ptr, ctx->dw.ptr,
+                        ctx->ll.ptr, ctx->off.ptr, ctx->zq.ptr, ctx->order.ptr, ctx->mono.ptr};
+    picca::PairView P{ctx->pa.ptr, ctx->pb.ptr, ctx->pang.ptr, ctx->pcs.ptr, ctx->pflags.ptr};
+    const int64_t ngroups = (int64_t)ctx->h_group_row.size();
+    picca::GroupView G{ctx->gbegin.ptr, ctx->grow.ptr, ngroups};
+    picca::XiOut O{ctx->o_we.ptr, ctx->o_xi.ptr, ctx->o_rp.ptr, ctx->o_rt.ptr, ctx->o_z.ptr,
+                   ctx->o_nb.ptr, ctx->queue.ptr, ctx->stats.ptr};
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->num_cus, ngroups));
+    PICCA_HIP_CHECK(hipEventRecord(ctx->ev_start, ctx->stream));
+    if (ngroups > 0) {
+        PICCA_HIP_CHECK(picca::launch_xi(mode, params->x_correlation != 0, C, F, P, G, O, grid,
+                                         lds, ctx->stream));
+    }
+    PICCA_HIP_CHECK(hipEventRecord(ctx->ev_stop, ctx->stream));
+    unsigned long long stats[picca::XI_NSTATS];
+    PICCA_HIP_CHECK(hipMemcpyAsync(stats, ctx->stats.ptr, sizeof(stats), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+    PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    PICCA_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    ctx->out_nb = C.nb;
+    ctx->out_rows = (int32_t)nrows;
+    ctx->have_result = true;
+    if (report) {
+        report->accepted_pairs = (int64_t)stats[picca::XI_STAT_ACCEPTED];
+        report->edge_pairs = (int64_t)stats[picca::XI_STAT_EDGE];
+        report->overflow_pairs = (int64_t)stats[picca::XI_STAT_OVERFLOW];
+        report->visited_pairs = (int64_t)stats[picca::XI_STAT_VISITED];
+        report->wave_steps = (int64_t)stats[picca::XI_STAT_STEPS];
+        report->kernel_ms = (double)ms;
+    }
+    return PICCA_OK;
+}
+
+int picca_xi_download(picca_ctx *ctx, double *weights, double *xi, double *r_par,
+                      double *r_trans, double *z, int64_t *num_pairs) {
+    if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
+    if (!ctx->have_result) return fail(PICCA_E_RUNTIME, "no xi result on the device");
+    PICCA_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)ctx->out_rows * (size_t)ctx->out_nb;
+    const size_t bytes = n * sizeof(double);
+    if (weights) PICCA_HIP_CHECK(hipMemcpyAsync(weights, ctx->o_we.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    if (xi) PICCA_HIP_CHECK(hipMemcpyAsync(xi, ctx->o_xi.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    if (r_par) PICCA_HIP_CHECK(hipMemcpyAsync(r_par, ctx->o_rp.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    if (r_trans) PICCA_HIP_CHECK(hipMemcpyAsync(r_trans, ctx->o_rt.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    if (z) PICCA_HIP_CHECK(hipMemcpyAsync(z, ctx->o_z.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    if (num_pairs)
+        PICCA_HIP_CHECK(hipMemcpyAsync(num_pairs, ctx->o_nb.ptr, n * sizeof(int64_t),
+                                       hipMemcpyDeviceToHost, ctx->stream));
+    PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return PICCA_OK;
+}
+
+int picca_dmat_run(picca_ctx *ctx, const picca_cf_params *params, int64_t num_pairs,
+                   const int32_t *pair_a, const int32_t *pair_b, const double *pair_ang,
+                   const uint8_t *pair_flags, double *weights_dmat, double *dmat,
+                   double *r_par_eff, double *r_trans_eff, double *z_eff,
+                   double *weight_eff, int64_t *stats) {
+    return fail(PICCA_E_RUNTIME, "picca_dmat_run is not available in this build");
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+"""Shared set-up of the correlation test cases.
+
+The cases replay the commands of picca's own regression tests
+(``py/picca/tests/test_3_cor.py``: test_cf :229-257, test_cf_image_data :259-286,
+test_cf_cross :458-488, test_cf_zcuts :791-823, test_cf_image_data_zcuts :825-856,
+test_cf_cross_zcuts :895-929, and the dmat tests :318-349, :490-524, :858-893,
+:931-968) against the golden files copied into tests/golden.
+"""
+import os
+
+import numpy as np
+
+from picca_amd import constants, io, utils
+from picca_amd.fitsio_lite import FITS
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+DELTA_LYA = os.path.join(GOLDEN, "Delta_LYA")
+DELTA_LYA_IMAGE = os.path.join(GOLDEN, "Delta_LYA_image")
+ATTRIBUTES = os.path.join(GOLDEN, "delta_attributes.fits.gz")
+
+COMMON = ["--rp-max", "+60.0", "--rt-max", "+60.0", "--nt", "15", "--nproc", "1",
+          "--in-attributes", ATTRIBUTES]
+ZCUTS = ["--z-min-pairs", "2.25", "--z-max-pairs", "2.3", "--z-min-sources", "2.3",
+         "--z-max-sources", "2.5"]
+
+CF_CASES = {
+    "cf": ["--in-dir", DELTA_LYA, "--rp-min", "+0.0", "--np", "15",
+           "--remove-same-half-plate-close-pairs"] + COMMON,
+    "cf_image": ["--in-dir", DELTA_LYA_IMAGE, "--rp-min", "+0.0", "--np", "15"] + COMMON,
+    "cf_cross": ["--in-dir", DELTA_LYA, "--in-dir2", DELTA_LYA, "--rp-min", "-60.0", "--np",
+                 "30", "--remove-same-half-plate-close-pairs", "--unfold-cf"] + COMMON,
+    "cf_zcuts": ["--in-dir", DELTA_LYA, "--rp-min", "+0.0", "--np", "15",
+                 "--remove-same-half-plate-close-pairs"] + ZCUTS + COMMON,
+    "cf_image_zcuts": ["--in-dir", DELTA_LYA_IMAGE, "--rp-min", "+0.0", "--np", "15"]
+                      + ZCUTS + COMMON,
+    "cf_cross_zcuts": ["--in-dir", DELTA_LYA, "--in-dir2", DELTA_LYA, "--rp-min", "-60.0",
+                       "--np", "30", "--remove-same-half-plate-close-pairs", "--unfold-cf"]
+                      + ZCUTS + COMMON,
+}
+
+DMAT_CASES = {
+    "dmat": ["--in-dir", DELTA_LYA, "--rp-min", "+0.0", "--np", "15", "--rej", "0.99",
+             "--remove-same-half-plate-close-pairs", "--no-redshift-evolution"] + COMMON,
+    "dmat_cross": ["--in-dir", DELTA_LYA, "--in-dir2", DELTA_LYA, "--rp-min", "-60.0",
+                   "--np", "30", "--rej", "0.99", "--remove-same-half-plate-close-pairs",
+                   "--unfold-cf", "--no-redshift-evolution"] + COMMON,
+    "dmat_zcuts": ["--in-dir", DELTA_LYA, "--rp-min", "+0.0", "--np", "15", "--rej", "0.99",
+                   "--remove-same-half-plate-close-pairs", "--no-redshift-evolution"]
+                  + ZCUTS + COMMON,
+    "dmat_cross_zcuts": ["--in-dir", DELTA_LYA, "--in-dir2", DELTA_LYA, "--rp-min", "-60.0",
+                         "--np", "30", "--rej", "0.99", "--remove-same-half-plate-close-pairs",
+                         "--unfold-cf", "--no-redshift-evolution"] + ZCUTS + COMMON,
+}
+
+
+def golden_path(name):
+    return os.path.join(GOLDEN, "cor", name + ".fits.gz")
+
+
+class Setup:
+    """Globals + data of one case, as the picca_cf / picca_dmat drivers set them."""
+
+    def __init__(self, argv, dmat=False):
+        args = {}
+        flags = set()
+        k = 0
+        while k < len(argv):
+            if k + 1 < len(argv) and not argv[k + 1].startswith("--"):
+                args[argv[k]] = argv[k + 1]
+                k += 2
+            else:
+                flags.add(argv[k])
+                k += 1
+        self.r_par_min = float(args.get("--rp-min", 0.0))
+        self.r_par_max = float(args.get("--rp-max", 200.0))
+        self.r_trans_max = float(args.get("--rt-max", 200.0))
+        self.num_bins_r_par = int(args.get("--np", 50))
+        self.num_bins_r_trans = int(args.get("--nt", 50))
+        self.coef = int(args.get("--coef-binning-model", 1))
+        self.num_model_bins_r_par = self.num_bins_r_par * self.coef if dmat else None
+        self.num_model_bins_r_trans = self.num_bins_r_trans * self.coef if dmat else None
+        self.z_min_pairs = float(args["--z-min-pairs"]) if "--z-min-pairs" in args else None
+        self.z_max_pairs = float(args["--z-max-pairs"]) if "--z-max-pairs" in args else None
+        self.zerr_cut_deg = None
+        self.zerr_cut_kms = None
+        self.z_ref = 2.25
+        self.alpha = 2.9
+        self.alpha2 = 2.9 if dmat else None
+        self.reject = float(args.get("--rej", 1.0))
+        self.x_correlation = False
+        self.rmu_binning = False
+        self.ang_correlation = False
+        self.remove_same_half_plate_close_pairs = "--remove-same-half-plate-close-pairs" in flags
+        self.redshift_evolution_in_distortion_matrix = "--no-redshift-evolution" not in flags
+        z_min_qso = float(args.get("--z-min-sources", 0.0))
+        z_max_qso = float(args.get("--z-max-sources", 10.0))
+        cosmo = constants.Cosmo(Om=0.315, verbose=False)
+        self.data, self.num_data, z_min, _ = io.read_deltas(
+            args["--in-dir"], 16, constants.ABSORBER_IGM["LYA"], 2.9, 2.25, cosmo,
+            z_min_qso=z_min_qso, z_max_qso=z_max_qso, delta_attributes=ATTRIBUTES)
+        self.ang_max = utils.compute_ang_max(cosmo, self.r_trans_max, z_min)
+        self.data2 = None
+        if "--in-dir2" in args:
+            if "--unfold-cf" in flags:
+                self.x_correlation = True
+            self.alpha2 = 2.9
+            self.data2, _, z_min2, _ = io.read_deltas(
+                args["--in-dir2"], 16, constants.ABSORBER_IGM["LYA"], 2.9, 2.25, cosmo,
+                z_min_qso=z_min_qso, z_max_qso=z_max_qso, delta_attributes=ATTRIBUTES)
+            self.ang_max = utils.compute_ang_max(cosmo, self.r_trans_max, z_min, z_min2)
+
+
+def reduce_rows(rows):
+    """The parent-process reduction of py/picca/bin/picca_cf.py:466-484."""
+    rows = np.array(rows)
+    we = rows[:, 0, :]
+    sw = we.sum(axis=0)
+    w = sw > 0
+    out = {}
+    for name, k in (("RP", 2), ("RT", 3), ("Z", 4)):
+        v = (rows[:, k, :] * we).sum(axis=0)
+        v[w] /= sw[w]
+        out[name] = v
+    out["NB"] = rows[:, 5, :].astype(np.int64).sum(axis=0)
+    out["WE"] = we
+    out["DA"] = rows[:, 1, :]
+    return out
+
+
+def golden_columns(name):
+    g = FITS(golden_path(name))
+    out = {}
+    for hdu in g.hdus[1:]:
+        for col in hdu.get_colnames():
+            out[col] = hdu[col][:]
+    return out
+
+
+def compare_to_golden(result, golden, rtol=1e-5, atol=1e-8, exact=("NB", "HEALPID")):
+    """compare_fits of py/picca/tests/test_helpers.py:45-112, column by column.
+
+    Returns {column: max relative difference}; raises AssertionError on failure.
+    """
+    report = {}
+    for col, ref in golden.items():
+        if col not in result:
+            continue
+        mine = np.asarray(result[col])
+        assert mine.shape == ref.shape, (col, mine.shape, ref.shape)
+        if col in exact:
+            assert np.array_equal(mine, ref), col
+            report[col] = 0.0
+            continue
+        assert np.allclose(mine, ref, rtol=rtol, atol=atol), col
+        denom = np.where(ref != 0, np.abs(ref), 1.0)
+        report[col] = float(np.max(np.abs(mine - ref) / denom)) if ref.size else 0.0
+    return report

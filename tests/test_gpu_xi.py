@@ -1,0 +1,339 @@
+"""Parity of the HIP pair kernel with the CPU oracle and with picca's golden files.
+
+Bar: num_pairs bit-exact; weights / xi / r_par / r_trans / z within 1e-6 relative after
+normalisation (FP64 sums accumulated in a different order); edge and overflow counts
+identical to the oracle's.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import cor_cases
+from oracle import cf_oracle as O
+from picca_amd import batch, cf, synth
+from picca_amd.engine import params_from_globals
+from picca_amd.fitsio_lite import FITS
+from picca_amd.store import ForestStore
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-6
+
+
+def set_globals(setup):
+    for name in ("num_bins_r_par", "num_bins_r_trans", "num_model_bins_r_par",
+                 "num_model_bins_r_trans", "r_par_min", "r_par_max", "r_trans_max",
+                 "z_min_pairs", "z_max_pairs", "zerr_cut_deg", "zerr_cut_kms", "z_ref", "alpha",
+                 "alpha2", "x_correlation", "rmu_binning", "ang_correlation",
+                 "remove_same_half_plate_close_pairs", "ang_max", "data", "data2", "reject",
+                 "redshift_evolution_in_distortion_matrix"):
+        setattr(cf, name, getattr(setup, name))
+    cf.counter = None
+    cf.lock = None
+
+
+def normalise(we, arr):
+    out = arr.copy()
+    mask = we > 0
+    out[mask] /= we[mask]
+    return out
+
+
+def assert_rows_match(gpu, oracle_rows):
+    """gpu: list of normalised row tuples; oracle_rows: list of normalised row tuples."""
+    for g, o in zip(gpu, oracle_rows):
+        assert np.array_equal(g[5], o[5])
+        for k in range(5):
+            np.testing.assert_allclose(g[k], o[k], rtol=RTOL, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", sorted(cor_cases.CF_CASES))
+def test_fixture_rows_match_oracle(case):
+    setup = cor_cases.Setup(cor_cases.CF_CASES[case])
+    set_globals(setup)
+    rows = cf.compute_xi_rows(sorted(setup.data))
+    report = cf.last_report
+    params = O.params_from_module(setup)
+    neigh = O.brute_force_neighbours(setup.data, setup.data2, sorted(setup.data),
+                                     setup.ang_max)
+    oracle_rows = []
+    stats = np.zeros(3, dtype=np.int64)
+    for hp in sorted(setup.data):
+        res, st = O.compute_xi(params, setup.data, [hp], neigh)
+        oracle_rows.append(res)
+        stats += st
+    assert_rows_match(rows, oracle_rows)
+    assert report["accepted_pairs"] == stats[0]
+    assert report["edge_pairs"] == stats[1]
+    assert report["overflow_pairs"] == stats[2]
+
+
+@pytest.mark.parametrize("case", sorted(cor_cases.CF_CASES))
+def test_driver_reproduces_golden(case, tmp_path):
+    from picca_amd.bin import picca_cf
+    out = str(tmp_path / (case + ".fits.gz"))
+    picca_cf.main(cor_cases.CF_CASES[case] + ["--out", out])
+    mine = FITS(out)
+    golden = FITS(cor_cases.golden_path(case))
+    assert len(mine) == len(golden)
+    result = {}
+    for hdu in mine.hdus[1:]:
+        assert hdu.get_colnames() == golden[hdu.get_extname()].get_colnames()
+        for col in hdu.get_colnames():
+            result[col] = hdu[col][:]
+    cor_cases.compare_to_golden(result, cor_cases.golden_columns(case))
+
+
+def test_compute_xi_single_call_matches_oracle():
+    setup = cor_cases.Setup(cor_cases.CF_CASES["cf"])
+    set_globals(setup)
+    healpixs = sorted(setup.data)[2:4]
+    cf.fill_neighs(healpixs)
+    mine = cf.compute_xi(healpixs)
+    params = O.params_from_module(setup)
+    neigh = O.brute_force_neighbours(setup.data, None, healpixs, setup.ang_max)
+    ref, _ = O.compute_xi(params, setup.data, healpixs, neigh)
+    assert np.array_equal(mine[5], ref[5])
+    for k in range(5):
+        np.testing.assert_allclose(mine[k], ref[k], rtol=RTOL, atol=1e-12)
+    # neighbours are consumed, as in the reference
+    assert all(d.neighbours is None for hp in healpixs for d in setup.data[hp])
+
+
+def test_compute_xi_forest_pairs_fast_in_place():
+    setup = cor_cases.Setup(cor_cases.CF_CASES["cf"])
+    set_globals(setup)
+    params = O.params_from_module(setup)
+    hp = sorted(setup.data)[3]
+    d1 = setup.data[hp][0]
+    neigh = O.brute_force_neighbours(setup.data, None, [hp], setup.ang_max)
+    nb = 15 * 15
+    # first neighbour pair with accepted pixel pairs
+    for d1 in setup.data[hp]:
+        found = None
+        for d2 in neigh[id(d1)]:
+            probe = [np.zeros(nb) for _ in range(5)] + [np.zeros(nb, dtype=np.int64)]
+            O.xi_pair(params, d1.z, d1.r_comov, d1.dist_m, d1.weights, d1.delta, d1.z_qso,
+                      d2.z, d2.r_comov, d2.dist_m, d2.weights, d2.delta, d2.z_qso,
+                      O.angle(d1, d2), False, probe, np.zeros(3, dtype=np.int64))
+            if probe[5].sum() > 0:
+                found = d2
+                break
+        if found is not None:
+            d2 = found
+            break
+    mine = [np.zeros(nb) for _ in range(5)] + [np.zeros(nb, dtype=np.int64)]
+    ref = [np.zeros(nb) for _ in range(5)] + [np.zeros(nb, dtype=np.int64)]
+    ang = O.angle(d1, d2)
+    for arrays in (mine, ref):
+        arrays[0] += 1.0  # in-place accumulation on top of existing content
+    cf.compute_xi_forest_pairs_fast(d1.z, d1.r_comov, d1.dist_m, d1.weights, d1.delta, d1.z_qso,
+                                    d2.z, d2.r_comov, d2.dist_m, d2.weights, d2.delta, d2.z_qso,
+                                    ang, False, *mine)
+    O.xi_pair(params, d1.z, d1.r_comov, d1.dist_m, d1.weights, d1.delta, d1.z_qso, d2.z,
+              d2.r_comov, d2.dist_m, d2.weights, d2.delta, d2.z_qso, ang, False, ref,
+              np.zeros(3, dtype=np.int64))
+    assert np.array_equal(mine[5], ref[5])
+    assert mine[5].sum() > 0
+    for k in range(5):
+        np.testing.assert_allclose(mine[k], ref[k], rtol=1e-12, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# synthetic catalogues, all modes
+# ---------------------------------------------------------------------------
+
+class Globals:
+    def __init__(self, **kw):
+        self.num_bins_r_par = 50
+        self.num_bins_r_trans = 50
+        self.num_model_bins_r_par = 0
+        self.num_model_bins_r_trans = 0
+        self.r_par_min = 0.0
+        self.r_par_max = 200.0
+        self.r_trans_max = 200.0
+        self.z_min_pairs = None
+        self.z_max_pairs = None
+        self.zerr_cut_deg = None
+        self.zerr_cut_kms = None
+        self.z_ref = 2.25
+        self.alpha = 2.9
+        self.alpha2 = 2.9
+        self.x_correlation = False
+        self.rmu_binning = False
+        self.ang_correlation = False
+        self.remove_same_half_plate_close_pairs = False
+        self.redshift_evolution_in_distortion_matrix = True
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+
+def run_both(store, pairs, g, device=0):
+    params = params_from_globals(g)
+    res = batch.run_xi(store, pairs, params, device)
+    oparams = O.params_from_module(g)
+    out, stats = O.xi_rows(oparams, store, pairs.pair_a, pairs.pair_b, pairs.pair_ang,
+                           pairs.pair_flags, pairs.pair_row, pairs.num_rows, num_threads=8)
+    return res, out, stats
+
+
+def check(res, out, stats):
+    assert np.array_equal(res.num_pairs, out[5])
+    assert res.report["accepted_pairs"] == stats[0] == out[5].sum()
+    assert res.report["edge_pairs"] == stats[1]
+    assert res.report["overflow_pairs"] == stats[2]
+    we_o = out[0]
+    np.testing.assert_allclose(res.weights, we_o, rtol=RTOL, atol=1e-12)
+    for mine, ref in ((res.xi, out[1]), (res.r_par, out[2]), (res.r_trans, out[3]),
+                      (res.z, out[4])):
+        np.testing.assert_allclose(normalise(res.weights, mine), normalise(we_o, ref),
+                                   rtol=RTOL, atol=1e-10)
+
+
+@pytest.fixture(scope="module")
+def small_survey():
+    return synth.make_survey("survey_small", seed=0, scale=0.3)
+
+
+def test_synthetic_auto_50x50(small_survey):
+    s = small_survey
+    from picca_amd.constants import Cosmo
+    from picca_amd.utils import compute_ang_max
+    ang_max = compute_ang_max(Cosmo(0.315, verbose=False), 200.0, s.z_min)
+    pairs = batch.xi_pairs(s.store, s.healpix, ang_max)
+    assert pairs.num_pairs > 1000
+    res, out, stats = run_both(s.store, pairs, Globals())
+    check(res, out, stats)
+    assert res.report["visited_pairs"] <= pairs.evaluated_pixel_pairs(s.store)
+
+
+@pytest.mark.parametrize("mode", ["cross", "zcuts", "zerr", "rmu", "angular", "shp", "rpmin"])
+def test_synthetic_modes(small_survey, mode):
+    s = small_survey
+    from picca_amd.constants import Cosmo
+    from picca_amd.utils import compute_ang_max
+    ang_max = compute_ang_max(Cosmo(0.315, verbose=False), 100.0, s.z_min)
+    kw = dict(r_par_max=100.0, r_trans_max=100.0, num_bins_r_par=25, num_bins_r_trans=25)
+    store = s.store
+    auto = True
+    if mode == "cross":
+        kw.update(x_correlation=True, r_par_min=-100.0, num_bins_r_par=50)
+        auto = False
+    elif mode == "zcuts":
+        kw.update(z_min_pairs=2.3, z_max_pairs=2.6)
+    elif mode == "zerr":
+        kw.update(zerr_cut_deg=0.5, zerr_cut_kms=20000.0)
+    elif mode == "rmu":
+        kw.update(rmu_binning=True, r_par_min=0.0, r_par_max=1.0)
+    elif mode == "angular":
+        kw.update(ang_correlation=True, r_par_min=1.0, r_par_max=1.05, r_trans_max=0.02,
+                  num_bins_r_par=20, num_bins_r_trans=20)
+        ang_max = 0.02
+        wave = 10 ** store.log_lambda
+        store = ForestStore(store.offsets, store.z, wave, wave, store.weights, store.delta,
+                            store.z_qso, store.ra, store.dec, store.thingid,
+                            log_lambda=store.log_lambda)
+    elif mode == "shp":
+        kw.update(remove_same_half_plate_close_pairs=True)
+        plate = (np.arange(store.num_forests) // 7).astype(np.int64)
+        fiber = (np.arange(store.num_forests) * 37 % 1000).astype(np.int64)
+        store = ForestStore(store.offsets, store.z, store.r_comov, store.dist_m,
+                            store.weights, store.delta, store.z_qso, store.ra, store.dec,
+                            store.thingid, plate=plate, fiberid=fiber)
+    elif mode == "rpmin":
+        kw.update(r_par_min=20.0)
+    g = Globals(**kw)
+    pairs = batch.xi_pairs(store, s.healpix, ang_max, auto=auto,
+                           target_store_index=None if auto else np.arange(store.num_forests),
+                           target_healpix=None if auto else s.healpix,
+                           remove_same_half_plate_close_pairs=g.remove_same_half_plate_close_pairs)
+    if mode == "shp":
+        assert pairs.pair_flags.sum() > 0
+    res, out, stats = run_both(store, pairs, g)
+    check(res, out, stats)
+    assert res.report["accepted_pairs"] > 0
+
+
+def _tiny_store(forests, ra=None):
+    n = len(forests)
+    lengths = [len(f["r"]) for f in forests]
+    offsets = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
+    cat = lambda key: np.concatenate([np.asarray(f[key], dtype=float) for f in forests])
+    ra = np.linspace(0.01, 0.0101, n) if ra is None else ra
+    return ForestStore(offsets, cat("z"), cat("r"), cat("d"), cat("w"), cat("delta"),
+                       np.full(n, 3.0), ra, np.full(n, 0.01), np.arange(n))
+
+
+def _forest(rng, n, r0=3800.0, dr=1.0):
+    r = r0 + dr * np.arange(n)
+    return {"r": r, "d": r.copy(), "z": 2.0 + r / 1e4, "w": rng.uniform(0.5, 2, n),
+            "delta": rng.normal(0, 0.3, n)}
+
+
+def test_edge_cases_zero_weights_long_and_nonmonotone():
+    rng = np.random.default_rng(5)
+    f_long = _forest(rng, 3000, dr=0.3)
+    f_zero = _forest(rng, 200)
+    f_zero["w"][::3] = 0.0
+    f_one = _forest(rng, 1, r0=3850.0)
+    f_nonmono = _forest(rng, 150)
+    f_nonmono["r"] = f_nonmono["r"][rng.permutation(150)]
+    f_empty_w = _forest(rng, 80)
+    f_empty_w["w"][:] = 0.0
+    forests = [f_long, f_zero, f_one, f_nonmono, f_empty_w, _forest(rng, 700, dr=0.7)]
+    store = _tiny_store(forests)
+    n = len(forests)
+    a, b = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    keep = a != b
+    pair_a, pair_b = a[keep], b[keep]
+    ang = np.full(len(pair_a), 0.004)
+    row = (pair_a % 2).astype(np.int32)
+    order = np.argsort(row, kind="stable")
+    pairs = batch.XiPairs(pair_a[order], pair_b[order], ang[order],
+                          np.zeros(len(order), dtype=np.uint8), row[order], np.array([0, 1]))
+    for kw in (dict(), dict(x_correlation=True, r_par_min=-200.0, num_bins_r_par=100)):
+        res, out, stats = run_both(store, pairs, Globals(**kw))
+        check(res, out, stats)
+
+
+def test_empty_pair_list():
+    rng = np.random.default_rng(1)
+    store = _tiny_store([_forest(rng, 10), _forest(rng, 12)])
+    pairs = batch.XiPairs(np.zeros(0, dtype=np.int64), np.zeros(0, dtype=np.int64),
+                          np.zeros(0), np.zeros(0, dtype=np.uint8), np.zeros(0, dtype=np.int32),
+                          np.array([5]))
+    res = batch.run_xi(store, pairs, params_from_globals(Globals()))
+    assert res.num_pairs.sum() == 0 and res.weights.shape == (1, 2500)
+
+
+def test_bin_overflow_is_counted_not_written():
+    """With r_par_min = -200, r_par = 199.99999999999997 passes the range test but
+    (r_par - r_par_min) rounds to 400, so the bin index reaches np (picca writes out of
+    bounds there): the pair is counted as overflow and not accumulated."""
+    r_below = np.nextafter(200.0, 0.0)
+    assert np.floor((r_below + 200.0) / 400.0 * 100) == 100
+    # ang = 0: cos(ang/2) = 1, so r_par = r1 - r2 exactly
+    f1 = {"r": np.array([r_below, 50.0]), "d": np.array([10.0, 10.0]), "z": np.array([2.5, 2.5]),
+          "w": np.array([1.0, 2.0]), "delta": np.array([0.1, 0.2])}
+    f2 = {"r": np.array([0.0, 30.0]), "d": np.array([10.0, 10.0]), "z": np.array([2.5, 2.5]),
+          "w": np.array([1.0, 1.0]), "delta": np.array([0.2, 0.3])}
+    store = _tiny_store([f1, f2])
+    pairs = batch.XiPairs(np.array([0]), np.array([1]), np.array([0.0]), np.zeros(1, np.uint8),
+                          np.zeros(1, np.int32), np.array([0]))
+    g = Globals(x_correlation=True, r_par_min=-200.0, r_par_max=200.0, num_bins_r_par=100,
+                num_bins_r_trans=50)
+    res, out, stats = run_both(store, pairs, g)
+    check(res, out, stats)
+    assert stats[2] == 1 and res.report["overflow_pairs"] == 1
+    assert res.num_pairs.sum() == 3
+
+
+def test_repeatable_counts(small_survey):
+    s = small_survey
+    pairs = batch.xi_pairs(s.store, s.healpix, 0.03)
+    params = params_from_globals(Globals())
+    r1 = batch.run_xi(s.store, pairs, params)
+    r2 = batch.run_xi(s.store, pairs, params)
+    assert np.array_equal(r1.num_pairs, r2.num_pairs)
+    np.testing.assert_allclose(r1.xi, r2.xi, rtol=1e-12, atol=1e-13)
