@@ -74,6 +74,8 @@ typedef struct picca_xi_report {
     int64_t overflow_pairs;  /* bin index >= np*nt (picca writes out of bounds)    */
     int64_t visited_pairs;   /* pixel pairs the kernel evaluated after pruning     */
     int64_t wave_steps;      /* 64-lane pair evaluations issued                     */
+    int64_t tasks;           /* (pair, 64-pixel chunk of delta1) work items         */
+    int64_t empty_tasks;     /* work items whose pruned pixel window was empty      */
     double kernel_ms;        /* device time of the pair kernel (HIP events)         */
 } picca_xi_report;
 

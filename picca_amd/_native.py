@@ -58,6 +58,8 @@ class XiReport(ctypes.Structure):
         ("overflow_pairs", ctypes.c_int64),
         ("visited_pairs", ctypes.c_int64),
         ("wave_steps", ctypes.c_int64),
+        ("tasks", ctypes.c_int64),
+        ("empty_tasks", ctypes.c_int64),
         ("kernel_ms", ctypes.c_double),
     ]
 

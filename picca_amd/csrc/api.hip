@@ -58,6 +58,14 @@ struct DeviceBuffer {
         if (err != hipSuccess || n == 0) return err;
         return hipMemcpyAsync(ptr, host, n * sizeof(T), hipMemcpyHostToDevice, stream);
     }
+    // n values followed by `pad` zeros (kernels may read a few elements past the end)
+    hipError_t upload_padded(const T *host, size_t n, size_t pad, hipStream_t stream) {
+        hipError_t err = reserve(n + pad);
+        if (err != hipSuccess) return err;
+        if (n > 0) err = hipMemcpyAsync(ptr, host, n * sizeof(T), hipMemcpyHostToDevice, stream);
+        if (err != hipSuccess) return err;
+        return hipMemsetAsync(ptr + n, 0, pad * sizeof(T), stream);
+    }
 };
 
 // Runs body(begin, end) over [0, n) on up to hardware_concurrency threads.
@@ -92,10 +100,13 @@ struct picca_ctx {
     // forest store
     int64_t num_forests = 0;
     int64_t num_pixels = 0;
+    double max_abs_r = 0;   // largest |r_comov| and |dist_m| of the store: bound the
+    double max_abs_d = 0;   // rounding of the pre-scaled coordinates of the fast loop
     bool has_log_lambda = false;
     bool has_order = false;
     std::vector<int64_t> h_offsets;
     DeviceBuffer<double> z, r, d, w, dw, ll, zq;
+    DeviceBuffer<picca::PixelRecord> pix;
     DeviceBuffer<int64_t> off;
     DeviceBuffer<int32_t> order;
     DeviceBuffer<uint8_t> mono;
@@ -115,8 +126,8 @@ struct picca_ctx {
     int64_t out_nb = 0;
     int32_t out_rows = 0;
     bool have_result = false;
-    DeviceBuffer<double> o_we, o_xi, o_rp, o_rt, o_z;
-    DeviceBuffer<unsigned long long> o_nb, queue, stats;
+    DeviceBuffer<picca::XiBin> o_bins;
+    DeviceBuffer<unsigned long long> queue, stats;
 };
 
 namespace {
@@ -158,6 +169,8 @@ int build_xi_const(const picca_cf_params *p, picca::XiConst &C, int &mode) {
     // rounding of frac * width in the edge test.
     C.fast_p = p->edge_tol / C.width_p * (1.0 + 1e-6) + 1e-12;
     C.fast_t = p->edge_tol / C.width_t * (1.0 + 1e-6) + 1e-12;
+    C.half_p = 0.5 - C.fast_p;
+    C.half_t = 0.5 - C.fast_t;
     C.zmin = p->z_min_pairs;
     C.zmax = p->z_max_pairs;
     C.zerr_ang = p->zerr_cut_deg * M_PI / 180.0;
@@ -274,15 +287,39 @@ int picca_set_forests(picca_ctx *ctx, int64_t num_forests, const int64_t *offset
                                 (forest_monotone(dist_m + o, n) ? 2 : 0));
         }
     });
+    double max_r = 0, max_d = 0;
+    bool finite = true;
+    for (int64_t k = 0; k < npix; ++k) {
+        max_r = std::max(max_r, std::fabs(r_comov[k]));
+        max_d = std::max(max_d, std::fabs(dist_m[k]));
+        finite = finite && std::isfinite(z[k]) && std::isfinite(weights[k]) &&
+                 std::isfinite(delta[k]) && std::isfinite(dw[k]);
+    }
+    if (!finite || !std::isfinite(max_r) || !std::isfinite(max_d))
+        return fail(PICCA_E_VALUE, "forest arrays must be finite (z, r_comov, dist_m, "
+                                   "weights, delta, delta*weights)");
+    ctx->max_abs_r = max_r;
+    ctx->max_abs_d = max_d;
     ctx->have_result = false;
     ctx->num_pairs = 0;
     ctx->h_offsets.assign(offsets, offsets + num_forests + 1);
     PICCA_HIP_CHECK(ctx->off.upload(offsets, (size_t)num_forests + 1, ctx->stream));
-    PICCA_HIP_CHECK(ctx->z.upload(z, (size_t)npix, ctx->stream));
-    PICCA_HIP_CHECK(ctx->r.upload(r_comov, (size_t)npix, ctx->stream));
-    PICCA_HIP_CHECK(ctx->d.upload(dist_m, (size_t)npix, ctx->stream));
-    PICCA_HIP_CHECK(ctx->w.upload(weights, (size_t)npix, ctx->stream));
-    PICCA_HIP_CHECK(ctx->dw.upload(dw.data(), (size_t)npix, ctx->stream));
+    // the pair kernel reads pixel batches of four, up to three past a forest's end
+    const size_t pad = picca::PIXEL_PAD;
+    PICCA_HIP_CHECK(ctx->z.upload_padded(z, (size_t)npix, pad, ctx->stream));
+    PICCA_HIP_CHECK(ctx->r.upload_padded(r_comov, (size_t)npix, pad, ctx->stream));
+    PICCA_HIP_CHECK(ctx->d.upload_padded(dist_m, (size_t)npix, pad, ctx->stream));
+    PICCA_HIP_CHECK(ctx->w.upload_padded(weights, (size_t)npix, pad, ctx->stream));
+    PICCA_HIP_CHECK(ctx->dw.upload_padded(dw.data(), (size_t)npix, pad, ctx->stream));
+    {
+        std::vector<picca::PixelRecord> records((size_t)std::max<int64_t>(npix, 1));
+        parallel_for(npix, [&](int64_t lo, int64_t hi) {
+            for (int64_t k = lo; k < hi; ++k)
+                records[k] = picca::PixelRecord{r_comov[k], dist_m[k], z[k], weights[k], dw[k]};
+        });
+        PICCA_HIP_CHECK(ctx->pix.upload_padded(records.data(), (size_t)npix, pad, ctx->stream));
+        PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
     PICCA_HIP_CHECK(ctx->zq.upload(z_qso, (size_t)num_forests, ctx->stream));
     PICCA_HIP_CHECK(ctx->mono.upload(mono.data(), (size_t)num_forests, ctx->stream));
     ctx->has_log_lambda = log_lambda != nullptr;
@@ -362,53 +399,69 @@ int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_rep
     int mode = 0;
     int status = build_xi_const(params, C, mode);
     if (status != PICCA_OK) return status;
-    // LDS: histogram (5 FP64 + 1 u32 per bin) + one staging tile per wave
-    const size_t hist_bytes = (size_t)C.nb * 5 * sizeof(double) + (size_t)((C.nb + 1) / 2) * 8;
-    const size_t static_lds = 1024;
-    const int waves = picca::XI_BLOCK / 64;
-    size_t lds = 0;
-    for (int tile = 64; tile >= 8; tile /= 2) {
-        lds = hist_bytes + (size_t)waves * 5 * tile * sizeof(double);
-        C.tile_len = tile;
-        if ((int64_t)(lds + static_lds) <= ctx->lds_limit) break;
+    // The fast loop classifies pairs from cos*r1 - cos*r2 and sin*d1 + sin*d2 instead of
+    // (r1 - r2)*cos and (d1 + d2)*sin; both differ from the reference value by a few ulp
+    // of the largest coordinate.  Widen the margins around every bin edge and range limit
+    // by that bound (in bin units) so that every pair whose bin or range test could differ
+    // is re-evaluated exactly.
+    {
+        const double u = std::ldexp(1.0, -53);
+        const double err_p = 16.0 * u * ctx->max_abs_r * C.inv_p + 8.0 * u * (C.np_d + 1.0);
+        const double err_t = 16.0 * u * ctx->max_abs_d * C.inv_t + 8.0 * u * (C.nt_d + 1.0);
+        C.fast_p += err_p;
+        C.fast_t += err_t;
+        C.half_p = 0.5 - C.fast_p;
+        C.half_t = 0.5 - C.fast_t;
+        if (!(C.half_p > 0.25 && C.half_t > 0.25))
+            return fail(PICCA_E_VALUE, "edge_tol is too large compared with the bin width");
     }
-    if ((int64_t)(lds + static_lds) > ctx->lds_limit)
-        return fail(PICCA_E_VALUE, "np*nt = " + std::to_string(C.nb) +
-                                       " bins do not fit the LDS histogram (max " +
-                                       std::to_string((ctx->lds_limit - static_lds -
-                                                       waves * 5 * 8 * 8) / 44) + ")");
+    // LDS: histogram (48 B per bin) when it fits, plus one staging tile per wave
+    const size_t static_lds = 1024;
+    const size_t waves = picca::XI_BLOCK / 64;
+    auto lds_bytes = [&](bool with_hist, int tile) {
+        return (with_hist ? (size_t)C.nb * picca::XI_HIST_BIN_BYTES : 0) +
+               waves * (size_t)(tile + 1) * picca::XI_TILE_PIXEL_BYTES;
+    };
+    bool lds_hist = false;
+    size_t lds = 0;
+    for (int tile = 64; tile >= 16 && !lds_hist; tile /= 2) {
+        if ((int64_t)(lds_bytes(true, tile) + static_lds) <= ctx->lds_limit) {
+            lds_hist = true;
+            C.tile_len = tile;
+            lds = lds_bytes(true, tile);
+        }
+    }
+    if (!lds_hist) {  // histogram too large for LDS: runs go straight to the HBM rows
+        C.tile_len = 64;
+        lds = lds_bytes(false, 64);
+    }
+    // the specialised kernel: standard binning, auto-correlation, r_par_min == 0, no cuts
+    const bool general = mode != picca::MODE_STANDARD || params->x_correlation ||
+                         params->r_par_min != 0.0 || params->has_z_min_pairs ||
+                         params->has_z_max_pairs || params->has_zerr_cut ||
+                         params->remove_same_half_plate_close_pairs;
     PICCA_HIP_CHECK(hipSetDevice(ctx->device));
     const int64_t nrows = ctx->num_rows;
     const int64_t nout = nrows * C.nb;
-    PICCA_HIP_CHECK(ctx->o_we.reserve((size_t)nout));
-    PICCA_HIP_CHECK(ctx->o_xi.reserve((size_t)nout));
-    PICCA_HIP_CHECK(ctx->o_rp.reserve((size_t)nout));
-    PICCA_HIP_CHECK(ctx->o_rt.reserve((size_t)nout));
-    PICCA_HIP_CHECK(ctx->o_z.reserve((size_t)nout));
-    PICCA_HIP_CHECK(ctx->o_nb.reserve((size_t)nout));
+    PICCA_HIP_CHECK(ctx->o_bins.reserve((size_t)nout));
     PICCA_HIP_CHECK(ctx->queue.reserve(1));
     PICCA_HIP_CHECK(ctx->stats.reserve(picca::XI_NSTATS));
-    PICCA_HIP_CHECK(picca::launch_zero(ctx->o_we.ptr, nout, ctx->stream));
-    PICCA_HIP_CHECK(picca::launch_zero(ctx->o_xi.ptr, nout, ctx->stream));
-    PICCA_HIP_CHECK(picca::launch_zero(ctx->o_rp.ptr, nout, ctx->stream));
-    PICCA_HIP_CHECK(picca::launch_zero(ctx->o_rt.ptr, nout, ctx->stream));
-    PICCA_HIP_CHECK(picca::launch_zero(ctx->o_z.ptr, nout, ctx->stream));
-    PICCA_HIP_CHECK(picca::launch_zero_u64(ctx->o_nb.ptr, nout, ctx->stream));
+    PICCA_HIP_CHECK(picca::launch_zero_u64(reinterpret_cast<unsigned long long *>(ctx->o_bins.ptr),
+                                           nout * (int64_t)(sizeof(picca::XiBin) / 8), ctx->stream));
     PICCA_HIP_CHECK(picca::launch_zero_u64(ctx->queue.ptr, 1, ctx->stream));
     PICCA_HIP_CHECK(picca::launch_zero_u64(ctx->stats.ptr, picca::XI_NSTATS, ctx->stream));
 
-    picca::ForestView F{ctx->z.ptr, ctx->r.ptr, ctx->d.ptr, ctx->w.ptr, ctx->dw.ptr,
+    picca::ForestView F{ctx->pix.ptr, ctx->z.ptr, ctx->r.ptr, ctx->d.ptr, ctx->w.ptr, ctx->dw.ptr,
                         ctx->ll.ptr, ctx->off.ptr, ctx->zq.ptr, ctx->order.ptr, ctx->mono.ptr};
     picca::PairView P{ctx->pa.ptr, ctx->pb.ptr, ctx->pang.ptr, ctx->pcs.ptr, ctx->pflags.ptr};
     const int64_t ngroups = (int64_t)ctx->h_group_row.size();
     picca::GroupView G{ctx->gbegin.ptr, ctx->grow.ptr, ngroups};
-    picca::XiOut O{ctx->o_we.ptr, ctx->o_xi.ptr, ctx->o_rp.ptr, ctx->o_rt.ptr, ctx->o_z.ptr,
-                   ctx->o_nb.ptr, ctx->queue.ptr, ctx->stats.ptr};
+    picca::XiOut O{ctx->o_bins.ptr, ctx->queue.ptr, ctx->stats.ptr};
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->num_cus, ngroups));
     PICCA_HIP_CHECK(hipEventRecord(ctx->ev_start, ctx->stream));
     if (ngroups > 0) {
-        PICCA_HIP_CHECK(picca::launch_xi(mode, params->x_correlation != 0, C, F, P, G, O, grid,
-                                         lds, ctx->stream));
+        PICCA_HIP_CHECK(picca::launch_xi(mode, params->x_correlation != 0, general, lds_hist, C,
+                                         F, P, G, O, grid, lds, ctx->stream));
     }
     PICCA_HIP_CHECK(hipEventRecord(ctx->ev_stop, ctx->stream));
     unsigned long long stats[picca::XI_NSTATS];
@@ -426,6 +479,8 @@ int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_rep
         report->overflow_pairs = (int64_t)stats[picca::XI_STAT_OVERFLOW];
         report->visited_pairs = (int64_t)stats[picca::XI_STAT_VISITED];
         report->wave_steps = (int64_t)stats[picca::XI_STAT_STEPS];
+        report->tasks = (int64_t)stats[picca::XI_STAT_TASKS];
+        report->empty_tasks = (int64_t)stats[picca::XI_STAT_EMPTY_TASKS];
         report->kernel_ms = (double)ms;
     }
     return PICCA_OK;
@@ -437,16 +492,20 @@ int picca_xi_download(picca_ctx *ctx, double *weights, double *xi, double *r_par
     if (!ctx->have_result) return fail(PICCA_E_RUNTIME, "no xi result on the device");
     PICCA_HIP_CHECK(hipSetDevice(ctx->device));
     const size_t n = (size_t)ctx->out_rows * (size_t)ctx->out_nb;
-    const size_t bytes = n * sizeof(double);
-    if (weights) PICCA_HIP_CHECK(hipMemcpyAsync(weights, ctx->o_we.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    if (xi) PICCA_HIP_CHECK(hipMemcpyAsync(xi, ctx->o_xi.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    if (r_par) PICCA_HIP_CHECK(hipMemcpyAsync(r_par, ctx->o_rp.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    if (r_trans) PICCA_HIP_CHECK(hipMemcpyAsync(r_trans, ctx->o_rt.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    if (z) PICCA_HIP_CHECK(hipMemcpyAsync(z, ctx->o_z.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    if (num_pairs)
-        PICCA_HIP_CHECK(hipMemcpyAsync(num_pairs, ctx->o_nb.ptr, n * sizeof(int64_t),
+    std::vector<picca::XiBin> host(std::max<size_t>(n, 1));
+    if (n > 0)
+        PICCA_HIP_CHECK(hipMemcpyAsync(host.data(), ctx->o_bins.ptr, n * sizeof(picca::XiBin),
                                        hipMemcpyDeviceToHost, ctx->stream));
     PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (size_t k = 0; k < n; ++k) {
+        const picca::XiBin &b = host[k];
+        if (weights) weights[k] = b.we;
+        if (xi) xi[k] = b.xi;
+        if (r_par) r_par[k] = b.rp;
+        if (r_trans) r_trans[k] = b.rt;
+        if (z) z[k] = b.z;
+        if (num_pairs) num_pairs[k] = (int64_t)b.nb;
+    }
     return PICCA_OK;
 }
 

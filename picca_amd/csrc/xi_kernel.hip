@@ -1,33 +1,36 @@
 // Pair kernel of the 3D correlation function on gfx950 (MI355X).
 //
 // GPU restatement of compute_xi_forest_pairs_fast (py/picca/cf.py:250-387) for a whole
-// batch of forest pairs, grouped by output row (healpix of delta1, as compute_xi is called
-// per healpix in py/picca/bin/picca_cf.py:452-458).
+// batch of forest pairs, grouped by output row (the healpix of delta1: picca_cf calls
+// compute_xi once per healpix, py/picca/bin/picca_cf.py:452-458).
 //
 // Work decomposition
-//   * one persistent workgroup of 1024 threads (16 wave64) per CU; each keeps a private
-//     histogram of the current output row in LDS: 5 FP64 sums + 1 u32 count per bin
-//     (44 B/bin, 110 KB at 50x50 bins) and flushes it to HBM with FP64 atomics when the
-//     row changes;
+//   * one persistent workgroup of 1024 threads (16 wave64) per CU keeps the histogram of
+//     its current output row in LDS, 48 B per bin (5 FP64 sums + u32 count, 120 KB at
+//     50x50 bins), and adds it to the row in HBM with FP64 atomics when the row changes.
+//     When np*nt does not fit in LDS the runs are added straight to the row in HBM;
 //   * workgroups pull "groups" (<= 64 consecutive pairs of one row) from an HBM queue;
-//   * inside a group, waves pull tasks = (pair, chunk of 64 pixels of delta1) from an LDS
-//     counter.  Lane l of the wave owns pixel i = 64*chunk + l of delta1 and keeps it in
-//     registers; the wave walks pixel j of delta2 in lockstep, so the values of pixel j
-//     are wave-uniform (scalar loads);
-//   * for the standard (r_par, r_trans) binning the j range is pruned to the window
-//     where |r_par| < r_par_max and r_trans < r_trans_max can hold for some lane
-//     (r_comov and dist_m are non-decreasing along a forest; forests where they are not
-//     are walked in full).  The skipped pixel pairs are provably rejected by the
-//     reference's range test;
+//   * inside a group, waves pull tasks = (pair, chunk of 64 pixels of delta1) from an
+//     LDS counter.  Lane l owns pixel i = 64*chunk + l of delta1 in registers; the wave
+//     walks the pixels j of delta2 in lockstep;
+//   * pixels j are staged per wave through LDS: lane l loads pixel j0 + l (coalesced),
+//     pixels the reference skips before any arithmetic (w2 == 0, zerr cut) are dropped,
+//     the rest compacted, and the next tile is already loading while the wave walks the
+//     current one with wave-uniform (broadcast) LDS reads;
+//   * for the standard (r_par, r_trans) binning the j range is pruned to the window where
+//     |r_par| < r_par_max and r_trans < r_trans_max can hold for some lane (r_comov and
+//     dist_m are non-decreasing along a forest; forests where they are not are walked in
+//     full).  The skipped pixel pairs are provably rejected by the reference's range test;
 //   * a lane accumulates consecutive accepted pairs that land in the same bin in
 //     registers (r_par and r_trans move monotonically with j) and adds the run to the
-//     LDS histogram with FP64 / u32 LDS atomics when its bin changes.
+//     histogram with six LDS atomics when its bin changes.
 //
-// Arithmetic: r_par, r_trans and the bin index use the reference's FP64 operations in
-// the reference's order (no contraction: built with -ffp-contract=off).  The bin index is
-// first taken with a reciprocal multiply; whenever the fractional part lies within the
-// edge tolerance (plus a rounding margin) of an integer it is recomputed with the
-// reference's divide-then-multiply, so every pair gets the reference's bin.
+// Arithmetic: r_par, r_trans and the bin index use the reference's FP64 operations in the
+// reference's order (built with -ffp-contract=off).  The bin index is first taken with a
+// reciprocal multiply; whenever the fractional part lies within the edge tolerance (plus
+// a rounding margin) of an integer it is recomputed with the reference's
+// divide-then-multiply, so every pair gets the reference's bin.  The mean pair redshift
+// (z1 + z2) / 2 is formed as z1/2 + z2/2, which is the same double.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -65,37 +68,20 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     return v;
 }
 
-// First index in [0, n) with arr[idx] >= x (n if none); arr must be non-decreasing.
-// All 64 lanes probe one level at a time, so the search takes ceil(log64 n) rounds.
-__device__ int wave_lower_bound(const double *__restrict__ arr, int n, double x, int lane) {
+// First index in [0, n) with arr[idx] >= x (n if none) when `strict`, or with
+// arr[idx] > x otherwise; arr must be non-decreasing.  All 64 lanes probe one level at a
+// time, so the search takes ceil(log64 n) rounds.
+template <bool STRICT>
+__device__ int wave_search(const double *__restrict__ arr, int n, double x, int lane) {
     int lo = 0, len = n;
     while (len > 0) {
-        int stride = (len + 63) >> 6;
-        int probe = lane * stride;
+        const int stride = (len + 63) >> 6;
+        const int probe = lane * stride;
         bool below = false;
-        if (probe < len) below = arr[lo + probe] < x;
-        int cnt = __popcll(__ballot(below));
+        if (probe < len) below = STRICT ? (arr[lo + probe] < x) : (arr[lo + probe] <= x);
+        const int cnt = __popcll(__ballot(below));
         if (cnt == 0) break;
-        int start = lo + (cnt - 1) * stride + 1;
-        int stop = lo + cnt * stride;
-        if (stop > lo + len) stop = lo + len;
-        lo = start;
-        len = stop - start;
-    }
-    return lo;
-}
-
-// First index in [0, n) with arr[idx] > x (n if none); arr must be non-decreasing.
-__device__ int wave_upper_bound(const double *__restrict__ arr, int n, double x, int lane) {
-    int lo = 0, len = n;
-    while (len > 0) {
-        int stride = (len + 63) >> 6;
-        int probe = lane * stride;
-        bool below = false;
-        if (probe < len) below = arr[lo + probe] <= x;
-        int cnt = __popcll(__ballot(below));
-        if (cnt == 0) break;
-        int start = lo + (cnt - 1) * stride + 1;
+        const int start = lo + (cnt - 1) * stride + 1;
         int stop = lo + cnt * stride;
         if (stop > lo + len) stop = lo + len;
         lo = start;
@@ -111,30 +97,177 @@ __device__ __forceinline__ double edge_distance(double t, double width) {
     return lo < hi ? lo : hi;
 }
 
-struct LaneAccumulator {
-    uint32_t bin;
+// histogram record of one bin in LDS
+struct LdsBin {
+    double we, xi, rp, rt, z;
+    uint32_t nb;
+    uint32_t pad;
+};
+static_assert(sizeof(LdsBin) == 48, "LdsBin layout");
+static_assert(sizeof(XiBin) == 48, "XiBin layout");
+
+// Staged pixel j of delta2.  The fast loop reads the 48-byte record (pre-scaled
+// coordinates, weights, z/2); the exact coordinates sit in a parallel 16-byte record that
+// only the exact re-evaluation near bin edges reads.
+struct TilePixel {
+    double cr, sd;  // cos(ang/2) * r_comov, sin(ang/2) * dist_m
+    double w, dw;   // weight, delta * weight
+    double zh;      // z / 2
+    double pad;
+};
+static_assert(sizeof(TilePixel) == 48, "TilePixel layout");
+
+struct TileExact {
+    double r, d;    // r_comov, dist_m
+};
+static_assert(sizeof(TileExact) == 16, "TileExact layout");
+
+struct LaneRun {
+    int32_t bin;
     uint32_t count;
     double we, xi, rp, rt, z;
 };
 
-__device__ __forceinline__ void flush_run(const LaneAccumulator &acc, double *h_we, double *h_xi,
-                                          double *h_rp, double *h_rt, double *h_z,
-                                          uint32_t *h_nb) {
-    __hip_atomic_fetch_add(h_we + acc.bin, acc.we, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    __hip_atomic_fetch_add(h_xi + acc.bin, acc.xi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    __hip_atomic_fetch_add(h_rp + acc.bin, acc.rp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    __hip_atomic_fetch_add(h_rt + acc.bin, acc.rt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    __hip_atomic_fetch_add(h_z + acc.bin, acc.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    __hip_atomic_fetch_add(h_nb + acc.bin, acc.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+// LDS-qualified view of the histogram: 32-bit addressing (a generic pointer costs a 64-bit
+// multiply-add per flush)
+typedef __attribute__((address_space(3))) LdsBin LocalBin;
+
+typedef __attribute__((address_space(3))) unsigned char LocalByte;
+
+__device__ __forceinline__ void flush_run(const LaneRun &run, LocalBin *hist) {
+    // bin < 2^24: one 24-bit multiply-add forms the byte address
+    LocalBin *b = (LocalBin *)((LocalByte *)hist +
+                               __umul24((uint32_t)run.bin, (uint32_t)sizeof(LdsBin)));
+    __hip_atomic_fetch_add(&b->we, run.we, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&b->xi, run.xi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&b->rp, run.rp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&b->rt, run.rt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&b->z, run.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&b->nb, run.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void flush_run(const LaneRun &run, XiBin *hist) {
+    XiBin *b = hist + run.bin;
+    unsafeAtomicAdd(&b->we, run.we);
+    unsafeAtomicAdd(&b->xi, run.xi);
+    unsafeAtomicAdd(&b->rp, run.rp);
+    unsafeAtomicAdd(&b->rt, run.rt);
+    unsafeAtomicAdd(&b->z, run.z);
+    atomicAdd(&b->nb, (unsigned long long)run.count);
+}
+
+// Reads one pixel record through the constant address space, so a wave-uniform address
+// becomes scalar loads.
+__device__ __forceinline__ PixelRecord load_record(const PixelRecord *p) {
+    typedef const __attribute__((address_space(4))) double ConstDouble;
+    const ConstDouble *q = (const ConstDouble *)p;
+    PixelRecord out;
+    out.r = q[0];
+    out.d = q[1];
+    out.z = q[2];
+    out.w = q[3];
+    out.dw = q[4];
+    return out;
+}
+
+struct LaneCounters {
+    uint64_t visit;
+    uint64_t accept;
+    uint32_t edge;
+    uint32_t overflow;
+};
+
+// Adds one accepted pair to the lane's run, flushing the run when the bin changes.  The
+// restart is a multiply by 0 instead of a branch that zeroes registers (the engine only
+// accepts finite inputs, so the sums are finite).
+template <typename Bin>
+__device__ __forceinline__ void add_pair(LaneRun &run, Bin *hist, LaneCounters &cnt, int bin,
+                                         double w12, double dw12, double r_par, double r_trans,
+                                         double z) {
+    const bool change = bin != run.bin;
+    if (change & (run.bin >= 0)) {
+        flush_run(run, hist);
+        cnt.accept += run.count;
+    }
+    const double keep = change ? 0.0 : 1.0;
+    run.bin = bin;
+    run.count = (change ? 0u : run.count) + 1u;
+    run.we = fma(run.we, keep, w12);
+    run.xi = fma(run.xi, keep, dw12);
+    run.rp = fma(r_par, w12, run.rp * keep);
+    run.rt = fma(r_trans, w12, run.rt * keep);
+    run.z = fma(z, w12, run.z * keep);
+}
+
+// Everything compute_xi_forest_pairs_fast does with one pixel pair after the weight and
+// zerr skips, with the reference's arithmetic (py/picca/cf.py:336-387): pair-redshift cut,
+// separations, range test, bins (reciprocal multiply, exact divide-then-multiply near an
+// edge), same-half-plate cut, overflow guard.  Returns true and sets bin / r_par / r_trans
+// for an accepted pair.
+template <int MODE, bool XCORR, bool GENERAL>
+__device__ __forceinline__ bool exact_pair(const XiConst &C, double r1, double d1, double r2,
+                                           double d2, double z, double ang, double cos_half,
+                                           double sin_half, bool same_half_plate, int &bin,
+                                           double &r_par, double &r_trans, uint32_t &edge,
+                                           uint32_t &overflow) {
+    if (GENERAL) {
+        if ((C.flags & XI_HAS_ZMIN) && z < C.zmin) return false;
+        if ((C.flags & XI_HAS_ZMAX) && z > C.zmax) return false;
+    }
+    if (MODE == MODE_ANGULAR) {
+        r_par = r1 / r2;
+        if (!XCORR && r_par < 1.0) r_par = 1.0 / r_par;
+        r_trans = ang;
+    } else {
+        r_par = (r1 - r2) * cos_half;
+        r_trans = (d1 + d2) * sin_half;
+        if (MODE == MODE_RMU) {
+            r_trans = sqrt(r_trans * r_trans + r_par * r_par);
+            r_par /= r_trans;
+        }
+        if (!XCORR) r_par = fabs(r_par);
+    }
+    if (r_par >= C.rpmax || r_trans >= C.rtmax || r_par < C.rpmin) {
+        // pairs just outside the range are edge pairs too
+        if ((r_par >= C.rpmax && r_par - C.rpmax < C.edge_tol) ||
+            (r_trans >= C.rtmax && r_trans - C.rtmax < C.edge_tol) ||
+            (r_par < C.rpmin && C.rpmin - r_par < C.edge_tol))
+            ++edge;
+        return false;
+    }
+    double t_par = (r_par - C.rpmin) * C.inv_p;
+    double t_trans = r_trans * C.inv_t;
+    double f_par = floor(t_par);
+    double f_trans = floor(t_trans);
+    if ((fabs(t_par - f_par - 0.5) > C.half_p) | (fabs(t_trans - f_trans - 0.5) > C.half_t)) {
+        t_par = (r_par - C.rpmin) / C.span_p * C.np_d;
+        t_trans = r_trans / C.rtmax * C.nt_d;
+        f_par = floor(t_par);
+        f_trans = floor(t_trans);
+        // an exact zero (equal r_comov, or ang = 0) cannot move under rounding
+        if ((r_par != 0 && edge_distance(t_par, C.width_p) < C.edge_tol) ||
+            (r_trans != 0 && edge_distance(t_trans, C.width_t) < C.edge_tol))
+            ++edge;
+    }
+    if (GENERAL && (C.flags & XI_REMOVE_SHP) && same_half_plate && fabs(r_par) < C.width_p)
+        return false;
+    const double bin_f = f_trans + C.nt_d * f_par;
+    if (!(bin_f >= 0.0 && bin_f < C.nb_d)) {
+        ++overflow;
+        return false;
+    }
+    bin = (int)bin_f;
+    return true;
 }
 
 // One task: pixels [64*chunk, 64*chunk + 64) of delta1 against the pruned window of delta2.
-template <int MODE, bool XCORR>
-__device__ void xi_task(const XiConst &C, const ForestView &F, int64_t p, int chunk, int lane,
-                        const PairView &P, double *h_we, double *h_xi, double *h_rp,
-                        double *h_rt, double *h_z, uint32_t *h_nb, uint64_t &n_visit,
-                        uint32_t &n_edge, uint32_t &n_overflow, uint64_t &n_steps,
-                        uint64_t &n_accept, double *tile, int tile_len) {
+//   GENERAL = false: standard binning, auto-correlation, r_par_min == 0, no z / zerr /
+//   same-half-plate cuts (the cuts of the reference that cannot trigger are compiled out).
+template <int MODE, bool XCORR, bool GENERAL, typename Bin>
+__device__ void xi_task(const XiConst &C, const XiConst &C_lds, const ForestView &F,
+                        const PairView &P, int64_t p, int chunk, int lane, Bin *hist,
+                        TilePixel *tile, TileExact *tile_x, LaneCounters &cnt,
+                        uint64_t &n_steps) {
     const int a = uniform_int(P.a[p]);
     const int b = uniform_int(P.b[p]);
     const int64_t oa = uniform_i64(F.off[a]);
@@ -144,29 +277,39 @@ __device__ void xi_task(const XiConst &C, const ForestView &F, int64_t p, int ch
     const double ang = uniform_f64(P.ang[p]);
     const double cos_half = uniform_f64(P.cs[2 * p]);
     const double sin_half = uniform_f64(P.cs[2 * p + 1]);
-    const bool same_half_plate = (uniform_int(P.flags[p]) & PAIR_SAME_HALF_PLATE) != 0;
-    const double z_qso_1 = uniform_f64(F.zq[a]);
-    const double z_qso_2 = uniform_f64(F.zq[b]);
-    const bool zerr_on = (C.flags & XI_HAS_ZERR) && (ang < C.zerr_ang);
+    bool same_half_plate = false;
+    bool zerr_on = false;
+    double z_qso_1 = 0, z_qso_2 = 0;
+    if (GENERAL) {
+        same_half_plate = (uniform_int(P.flags[p]) & PAIR_SAME_HALF_PLATE) != 0;
+        zerr_on = (C.flags & XI_HAS_ZERR) && (ang < C.zerr_ang);
+        z_qso_1 = uniform_f64(F.zq[a]);
+        z_qso_2 = uniform_f64(F.zq[b]);
+    }
 
     const int i = chunk * 64 + lane;
     bool active = i < la;
-    double r1 = 0, d1 = 0, z1 = 0, w1 = 0, dw1 = 0;
+    double r1 = 0, d1 = 0, zh1 = 0, w1 = 0, dw1 = 0;
     if (active) {
         r1 = F.r[oa + i];
         d1 = F.d[oa + i];
-        z1 = F.z[oa + i];
+        zh1 = F.z[oa + i] * 0.5;
         w1 = F.w[oa + i];
         dw1 = F.dw[oa + i];
         active = w1 != 0;
     }
-    if (zerr_on && active) {
+    if (GENERAL && zerr_on && active) {
+        const double z1 = zh1 * 2.0;
         double z_qf = 0.5 * (z1 + z_qso_2);
         double dv_kms = fabs(z1 - z_qso_2) / (1 + z_qf);
         dv_kms *= SPEED_LIGHT_KMS;
         if (dv_kms < C.zerr_kms) active = false;
     }
     if (__ballot(active) == 0) return;
+    // Drain the loads of this lane's pixel now: the pixel loop keeps tile loads in
+    // flight, and a value still pending here would otherwise be waited for with
+    // vmcnt(0) inside the loop.  (gfx9 s_waitcnt: vmcnt 0, expcnt 7, lgkmcnt 15.)
+    __builtin_amdgcn_s_waitcnt(0x0F70);
 
     const double *__restrict__ Br = F.r + ob;
     const double *__restrict__ Bd = F.d + ob;
@@ -182,45 +325,85 @@ __device__ void xi_task(const XiConst &C, const ForestView &F, int64_t p, int ch
         const double d_min = wave_min(active ? d1 : INFINITY);
         // r_par = (r1 - r2) cos_half must reach [r_par_min - tol, r_par_max + tol)
         const double slack = 4.0 * C.edge_tol / cos_half + 1e-6;
-        double lo_r, hi_r;
-        if (XCORR) {
-            lo_r = r_min - C.rpmax / cos_half;
-            hi_r = r_max - C.rpmin / cos_half;
-        } else {
-            lo_r = r_min - C.rpmax / cos_half;
-            hi_r = r_max + C.rpmax / cos_half;
-        }
+        double lo_r = r_min - C.rpmax / cos_half;
+        double hi_r = XCORR ? r_max - C.rpmin / cos_half : r_max + C.rpmax / cos_half;
         lo_r -= slack + 1e-12 * fabs(lo_r);
         hi_r += slack + 1e-12 * fabs(hi_r);
-        j_lo = wave_lower_bound(Br, lb, lo_r, lane);
-        j_hi = wave_upper_bound(Br, lb, hi_r, lane);
+        j_lo = wave_search<true>(Br, lb, lo_r, lane);
+        j_hi = wave_search<false>(Br, lb, hi_r, lane);
         if (sin_half > 0) {
             double d_lim = (C.rtmax + 4.0 * C.edge_tol) / sin_half - d_min;
             d_lim += 1e-6 + 1e-12 * fabs(d_lim);
-            int j_d = wave_upper_bound(Bd, lb, d_lim, lane);
+            const int j_d = wave_search<false>(Bd, lb, d_lim, lane);
             if (j_d < j_hi) j_hi = j_d;
         }
     }
     j_lo = uniform_int(j_lo);
     j_hi = uniform_int(j_hi);
 
-    LaneAccumulator acc;
-    acc.bin = 0xffffffffu;
-    acc.count = 0;
-    acc.we = acc.xi = acc.rp = acc.rt = acc.z = 0.0;
+    LaneRun run;
+    run.bin = -1;
+    run.count = 0;
+    run.we = run.xi = run.rp = run.rt = run.z = 0.0;
     uint32_t visit = 0;
-    uint32_t steps = 0;
+    uint32_t edge = 0;
+    uint32_t overflow = 0;
+    const int tile_len = C.tile_len;
 
-    // Pixel j of delta2 is staged through this wave's LDS tile: lane l loads pixel
-    // j0 + l (coalesced), pixels with w2 == 0 or failing the zerr cut are dropped
-    // (the reference `continue`s on them before anything else), the survivors are
-    // compacted, and the next tile's loads are already in flight while the wave
-    // walks the current one with wave-uniform LDS reads.
-    double *t_r = tile;
-    double *t_d = tile + tile_len;
-    double *t_z = tile + 2 * tile_len;
-    double *t_w = tile + 3 * tile_len;
-    double *t_dw = tile + 4 * tile_len;
+    if (!GENERAL) {
+        // Pixel j of delta2 is wave-uniform: its record comes through the scalar cache into
+        // SGPRs and the loop touches LDS only to flush runs.  Two pixels per iteration, the
+        // next two already loading; the store is padded, so reading up to three records
+        // past j_hi stays in bounds.
+        const PixelRecord *rec = F.pix + ob;
+        uint32_t steps = 0;
+        PixelRecord c0 = load_record(rec + j_lo), c1 = load_record(rec + j_lo + 1);
+        auto step = [&](const PixelRecord &q) {
+            // the reference skips w2 == 0 (either sign) before any arithmetic
+            if ((__double_as_longlong(q.w) << 1) == 0) return;
+            ++steps;
+            double r_par = (r1 - q.r) * cos_half;
+            const double r_trans = (d1 + q.d) * sin_half;
+            const double t_par = fabs(r_par) * C.inv_p;
+            const double t_trans = r_trans * C.inv_t;
+            const double f_par = floor(t_par);
+            const double f_trans = floor(t_trans);
+            // r_par == 0 exactly (equal r_comov) is bin 0 and never an edge pair
+            const bool near = ((fabs(t_par - f_par - 0.5) > C.half_p) & (r_par != 0)) |
+                              (fabs(t_trans - f_trans - 0.5) > C.half_t);
+            bool accept = active & !near & (t_par < C.np_d) & (t_trans < C.nt_d);
+            int bin = (int)__umul24((unsigned)C.nt, (unsigned)(int)f_par) + (int)f_trans;
+            double rp = r_par, rt = r_trans;
+            if (active & near) {
+                // exact re-evaluation (py/picca/cf.py:355-376)
+                // (constants from the LDS copy: this rare path keeps no SGPRs busy)
+                accept = exact_pair<MODE, XCORR, false>(C_lds, r1, d1, q.r, q.d, 0.0, ang,
+                                                        cos_half, sin_half, false, bin, rp, rt,
+                                                        edge, overflow);
+            }
+            if (accept)
+                add_pair(run, hist, cnt, bin, w1 * q.w, dw1 * q.dw, fabs(rp), rt,
+                         fma(q.z, 0.5, zh1));
+        };
+        for (int j = j_lo; j < j_hi; j += 2) {
+            const PixelRecord n0 = load_record(rec + j + 2), n1 = load_record(rec + j + 3);
+            step(c0);
+            if (j + 1 < j_hi) step(c1);
+            c0 = n0;
+            c1 = n1;
+        }
+        n_steps += steps;
+        if (active) visit += steps;
+        if (run.bin >= 0) {
+            flush_run(run, hist);
+            cnt.accept += run.count;
+        }
+        cnt.visit += visit;
+        cnt.edge += edge;
+        cnt.overflow += overflow;
+        return;
+    }
+
     double pr = 0, pd = 0, pz = 0, pw = 0, pdw = 0;
     bool pvalid = false;
     {
@@ -235,8 +418,9 @@ __device__ void xi_task(const XiConst &C, const ForestView &F, int64_t p, int ch
         }
     }
     for (int j0 = j_lo; j0 < j_hi; j0 += tile_len) {
+        // stage the prefetched tile: drop what the reference skips, compact the rest
         bool keep = pvalid && pw != 0;
-        if (zerr_on && keep) {
+        if (GENERAL && zerr_on && keep) {
             double z_qf = 0.5 * (pz + z_qso_1);
             double dv_kms = fabs(pz - z_qso_1) / (1 + z_qf);
             dv_kms *= SPEED_LIGHT_KMS;
@@ -247,11 +431,14 @@ __device__ void xi_task(const XiConst &C, const ForestView &F, int64_t p, int ch
         const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
         if (keep) {
-            t_r[pos] = pr;
-            t_d[pos] = pd;
-            t_z[pos] = pz;
-            t_w[pos] = pw;
-            t_dw[pos] = pdw;
+            TilePixel *t = tile + pos;
+            t->cr = cos_half * pr;
+            t->sd = sin_half * pd;
+            t->w = pw;
+            t->dw = pdw;
+            t->zh = pz * 0.5;
+            tile_x[pos].r = pr;
+            tile_x[pos].d = pd;
         }
         __builtin_amdgcn_wave_barrier();
         {
@@ -265,134 +452,71 @@ __device__ void xi_task(const XiConst &C, const ForestView &F, int64_t p, int ch
                 pdw = Bdw[j];
             }
         }
-        steps += n_tile;
-    for (int u = 0; u < n_tile; ++u) {
-        const double r2 = t_r[u];
-        const double d2 = t_d[u];
-        const double z2 = t_z[u];
-        const double w2 = t_w[u];
-        const double dw2 = t_dw[u];
-
-        bool ok = active;
-        const double z = (z1 + z2) / 2;
-        if ((C.flags & XI_HAS_ZMIN) && z < C.zmin) ok = false;
-        if ((C.flags & XI_HAS_ZMAX) && z > C.zmax) ok = false;
-
-        double r_par, r_trans;
-        if (MODE == MODE_ANGULAR) {
-            r_par = r1 / r2;
-            if (!XCORR && r_par < 1.0) r_par = 1.0 / r_par;
-            r_trans = ang;
-        } else {
-            r_par = (r1 - r2) * cos_half;
-            r_trans = (d1 + d2) * sin_half;
-            if (MODE == MODE_RMU) {
-                r_trans = sqrt(r_trans * r_trans + r_par * r_par);
-                r_par /= r_trans;
-            }
-            if (!XCORR) r_par = fabs(r_par);
-        }
-        visit += ok ? 1u : 0u;
-        const bool out = (r_par >= C.rpmax) | (r_trans >= C.rtmax) | (r_par < C.rpmin);
-        if (ok && out) {
-            if ((r_par >= C.rpmax && r_par - C.rpmax < C.edge_tol) ||
-                (r_trans >= C.rtmax && r_trans - C.rtmax < C.edge_tol) ||
-                (r_par < C.rpmin && C.rpmin - r_par < C.edge_tol))
-                ++n_edge;
-        }
-        if (ok && !out) {
-            double t_par = (r_par - C.rpmin) * C.inv_p;
-            double t_trans = r_trans * C.inv_t;
-            double f_par = floor(t_par);
-            double f_trans = floor(t_trans);
-            const double frac_p = t_par - f_par;
-            const double frac_t = t_trans - f_trans;
-            if (frac_p < C.fast_p || frac_p > 1.0 - C.fast_p || frac_t < C.fast_t ||
-                frac_t > 1.0 - C.fast_t) {
-                t_par = (r_par - C.rpmin) / C.span_p * C.np_d;
-                t_trans = r_trans / C.rtmax * C.nt_d;
-                f_par = floor(t_par);
-                f_trans = floor(t_trans);
-                // an exact zero (equal r_comov, or ang = 0) cannot move under rounding
-                if ((r_par != 0 && edge_distance(t_par, C.width_p) < C.edge_tol) ||
-                    (r_trans != 0 && edge_distance(t_trans, C.width_t) < C.edge_tol))
-                    ++n_edge;
-            }
-            bool keep = true;
-            if ((C.flags & XI_REMOVE_SHP) && same_half_plate && fabs(r_par) < C.width_p)
-                keep = false;
-            const double bin_f = f_trans + C.nt_d * f_par;
-            if (keep && !(bin_f >= 0.0 && bin_f < C.nb_d)) {
-                ++n_overflow;
-                keep = false;
-            }
-            if (keep) {
-                const uint32_t bin = (uint32_t)bin_f;
-                const double w12 = w1 * w2;
-                const double dw12 = dw1 * dw2;
-                if (bin != acc.bin) {
-                    if (acc.bin != 0xffffffffu) {
-                        flush_run(acc, h_we, h_xi, h_rp, h_rt, h_z, h_nb);
-                        n_accept += acc.count;
-                    }
-                    acc.bin = bin;
-                    acc.count = 1;
-                    acc.we = w12;
-                    acc.xi = dw12;
-                    acc.rp = r_par * w12;
-                    acc.rt = r_trans * w12;
-                    acc.z = z * w12;
-                } else {
-                    acc.count += 1;
-                    acc.we += w12;
-                    acc.xi += dw12;
-                    acc.rp += r_par * w12;
-                    acc.rt += r_trans * w12;
-                    acc.z += z * w12;
-                }
+        n_steps += n_tile;
+        {
+            for (int u = 0; u < n_tile; ++u) {
+                const double2 wdw = *reinterpret_cast<const double2 *>(&tile[u].w);
+                const double zh2 = tile[u].zh;
+                const double2 rd = *reinterpret_cast<const double2 *>(&tile_x[u].r);
+                const double z = zh1 + zh2;
+                if (!active) continue;
+                if (!((C.flags & XI_HAS_ZMIN) && z < C.zmin) &&
+                    !((C.flags & XI_HAS_ZMAX) && z > C.zmax))
+                    ++visit;
+                int bin;
+                double r_par, r_trans;
+                if (exact_pair<MODE, XCORR, GENERAL>(C, r1, d1, rd.x, rd.y, z, ang, cos_half,
+                                                     sin_half, same_half_plate, bin, r_par,
+                                                     r_trans, edge, overflow))
+                    add_pair(run, hist, cnt, bin, w1 * wdw.x, dw1 * wdw.y, r_par, r_trans, z);
             }
         }
-    }
         __builtin_amdgcn_wave_barrier();
     }
-    if (acc.bin != 0xffffffffu) {
-        flush_run(acc, h_we, h_xi, h_rp, h_rt, h_z, h_nb);
-        n_accept += acc.count;
+    if (run.bin >= 0) {
+        flush_run(run, hist);
+        cnt.accept += run.count;
     }
-    n_visit += visit;
-    n_steps += steps;
+    cnt.visit += visit;
+    cnt.edge += edge;
+    cnt.overflow += overflow;
 }
 
-template <int MODE, bool XCORR>
+template <int MODE, bool XCORR, bool GENERAL, bool LDS_HIST>
 __global__ void __launch_bounds__(XI_BLOCK)
 xi_kernel(XiConst C, ForestView F, PairView P, GroupView G, XiOut O) {
-    extern __shared__ double smem[];
+    extern __shared__ __align__(16) unsigned char smem[];
     const int nb = C.nb;
-    double *h_we = smem;
-    double *h_xi = h_we + nb;
-    double *h_rp = h_xi + nb;
-    double *h_rt = h_rp + nb;
-    double *h_z = h_rt + nb;
-    uint32_t *h_nb = reinterpret_cast<uint32_t *>(h_z + nb);
-    // per-wave staging tiles after the histogram (8-byte aligned)
-    double *tile = h_z + nb + (nb + 1) / 2 + (threadIdx.x >> 6) * 5 * C.tile_len;
+    LocalBin *hist = (LocalBin *)reinterpret_cast<LdsBin *>(smem);
+    const size_t hist_bytes = LDS_HIST ? (size_t)nb * sizeof(LdsBin) : 0;
+    // per wave: (tile_len + 1) staged pixels, then (tile_len + 1) exact coordinates
+    const int wave = threadIdx.x >> 6;
+    unsigned char *wave_tiles = smem + hist_bytes +
+                                (size_t)wave * (C.tile_len + 1) * XI_TILE_PIXEL_BYTES;
+    TilePixel *tile = reinterpret_cast<TilePixel *>(wave_tiles);
+    TileExact *tile_x = reinterpret_cast<TileExact *>(
+        wave_tiles + (size_t)(C.tile_len + 1) * sizeof(TilePixel));
 
     __shared__ int64_t s_group;
     __shared__ int s_task;
     __shared__ int s_prefix[XI_GROUP_PAIRS + 1];
     __shared__ unsigned long long s_stats[XI_NSTATS];
+    __shared__ XiConst s_const;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
 
-    for (int k = tid; k < nb; k += XI_BLOCK) {
-        h_we[k] = h_xi[k] = h_rp[k] = h_rt[k] = h_z[k] = 0.0;
-        h_nb[k] = 0;
+    if (LDS_HIST) {
+        for (int k = tid; k < nb; k += XI_BLOCK) {
+            hist[k].we = hist[k].xi = hist[k].rp = hist[k].rt = hist[k].z = 0.0;
+            hist[k].nb = 0;
+        }
     }
     if (tid < XI_NSTATS) s_stats[tid] = 0;
+    if (tid == 0) s_const = C;  // published by the first __syncthreads of the group loop
 
-    uint64_t n_visit = 0, n_steps = 0, n_accept = 0;
-    uint32_t n_edge = 0, n_overflow = 0;
+    LaneCounters cnt = {0, 0, 0, 0};
+    uint64_t n_steps = 0, n_tasks = 0, n_empty_tasks = 0;  // wave-uniform
     int current_row = -1;
 
     for (;;) {
@@ -402,25 +526,25 @@ xi_kernel(XiConst C, ForestView F, PairView P, GroupView G, XiOut O) {
         const int64_t g = s_group;
         if (g >= G.count) break;
         const int row = G.row[g];
-        if (row != current_row) {
+        if (LDS_HIST && row != current_row) {
             if (current_row >= 0) {
-                const int64_t base = (int64_t)current_row * nb;
+                XiBin *out = O.bins + (int64_t)current_row * nb;
                 for (int k = tid; k < nb; k += XI_BLOCK) {
-                    const uint32_t cnt = h_nb[k];
-                    if (cnt) {
-                        unsafeAtomicAdd(O.we + base + k, h_we[k]);
-                        unsafeAtomicAdd(O.xi + base + k, h_xi[k]);
-                        unsafeAtomicAdd(O.rp + base + k, h_rp[k]);
-                        unsafeAtomicAdd(O.rt + base + k, h_rt[k]);
-                        unsafeAtomicAdd(O.z + base + k, h_z[k]);
-                        atomicAdd(O.nb + base + k, (unsigned long long)cnt);
-                        h_we[k] = h_xi[k] = h_rp[k] = h_rt[k] = h_z[k] = 0.0;
-                        h_nb[k] = 0;
+                    const uint32_t n = hist[k].nb;
+                    if (n) {
+                        unsafeAtomicAdd(&out[k].we, hist[k].we);
+                        unsafeAtomicAdd(&out[k].xi, hist[k].xi);
+                        unsafeAtomicAdd(&out[k].rp, hist[k].rp);
+                        unsafeAtomicAdd(&out[k].rt, hist[k].rt);
+                        unsafeAtomicAdd(&out[k].z, hist[k].z);
+                        atomicAdd(&out[k].nb, (unsigned long long)n);
+                        hist[k].we = hist[k].xi = hist[k].rp = hist[k].rt = hist[k].z = 0.0;
+                        hist[k].nb = 0;
                     }
                 }
             }
-            current_row = row;
         }
+        current_row = row;
         const int64_t p0 = G.begin[g];
         const int npairs = (int)(G.begin[g + 1] - p0);
         // chunk prefix over the pairs of the group (one wave)
@@ -459,60 +583,56 @@ xi_kernel(XiConst C, ForestView F, PairView P, GroupView G, XiOut O) {
             }
             const int q = uniform_int(lo);
             const int chunk = uniform_int(t - s_prefix[q]);
-            xi_task<MODE, XCORR>(C, F, p0 + q, chunk, lane, P, h_we, h_xi, h_rp, h_rt, h_z,
-                                 h_nb, n_visit, n_edge, n_overflow, n_steps, n_accept, tile,
-                                 C.tile_len);
+            const uint64_t steps_before = n_steps;
+            if (LDS_HIST) {
+                xi_task<MODE, XCORR, GENERAL>(C, s_const, F, P, p0 + q, chunk, lane, hist, tile,
+                                              tile_x, cnt, n_steps);
+            } else {
+                xi_task<MODE, XCORR, GENERAL>(C, s_const, F, P, p0 + q, chunk, lane,
+                                              O.bins + (int64_t)row * nb, tile, tile_x, cnt,
+                                              n_steps);
+            }
+            ++n_tasks;
+            if (n_steps == steps_before) ++n_empty_tasks;
         }
     }
     // final flush (all waves are past the last __syncthreads of the loop)
-    if (current_row >= 0) {
-        const int64_t base = (int64_t)current_row * nb;
+    if (LDS_HIST && current_row >= 0) {
+        XiBin *out = O.bins + (int64_t)current_row * nb;
         for (int k = tid; k < nb; k += XI_BLOCK) {
-            const uint32_t cnt = h_nb[k];
-            if (cnt) {
-                unsafeAtomicAdd(O.we + base + k, h_we[k]);
-                unsafeAtomicAdd(O.xi + base + k, h_xi[k]);
-                unsafeAtomicAdd(O.rp + base + k, h_rp[k]);
-                unsafeAtomicAdd(O.rt + base + k, h_rt[k]);
-                unsafeAtomicAdd(O.z + base + k, h_z[k]);
-                atomicAdd(O.nb + base + k, (unsigned long long)cnt);
+            const uint32_t n = hist[k].nb;
+            if (n) {
+                unsafeAtomicAdd(&out[k].we, hist[k].we);
+                unsafeAtomicAdd(&out[k].xi, hist[k].xi);
+                unsafeAtomicAdd(&out[k].rp, hist[k].rp);
+                unsafeAtomicAdd(&out[k].rt, hist[k].rt);
+                unsafeAtomicAdd(&out[k].z, hist[k].z);
+                atomicAdd(&out[k].nb, (unsigned long long)n);
             }
         }
     }
     // statistics: lane values -> wave sums -> workgroup -> HBM
-    const uint64_t v_visit = wave_sum_u64(n_visit);
-    const uint64_t v_edge = wave_sum_u64(n_edge);
-    const uint64_t v_over = wave_sum_u64(n_overflow);
-    const uint64_t v_accept = wave_sum_u64(n_accept);
+    const uint64_t v_visit = wave_sum_u64(cnt.visit);
+    const uint64_t v_edge = wave_sum_u64(cnt.edge);
+    const uint64_t v_over = wave_sum_u64(cnt.overflow);
+    const uint64_t v_accept = wave_sum_u64(cnt.accept);
     if (lane == 0) {
         atomicAdd(&s_stats[XI_STAT_VISITED], (unsigned long long)v_visit);
         atomicAdd(&s_stats[XI_STAT_EDGE], (unsigned long long)v_edge);
         atomicAdd(&s_stats[XI_STAT_OVERFLOW], (unsigned long long)v_over);
         atomicAdd(&s_stats[XI_STAT_STEPS], (unsigned long long)n_steps);
         atomicAdd(&s_stats[XI_STAT_ACCEPTED], (unsigned long long)v_accept);
+        atomicAdd(&s_stats[XI_STAT_TASKS], (unsigned long long)n_tasks);
+        atomicAdd(&s_stats[XI_STAT_EMPTY_TASKS], (unsigned long long)n_empty_tasks);
     }
     __syncthreads();
     if (tid < XI_NSTATS) atomicAdd(O.stats + tid, s_stats[tid]);
-}
-
-__global__ void zero_kernel(double *__restrict__ a, int64_t n) {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (; k < n; k += stride) a[k] = 0.0;
 }
 
 __global__ void zero_u64_kernel(unsigned long long *__restrict__ a, int64_t n) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (; k < n; k += stride) a[k] = 0ull;
-}
-
-hipError_t launch_zero(double *ptr, int64_t n, hipStream_t stream) {
-    if (n <= 0) return hipSuccess;
-    int64_t blocks = (n + 255) / 256;
-    if (blocks > 4096) blocks = 4096;
-    zero_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(ptr, n);
-    return hipGetLastError();
 }
 
 hipError_t launch_zero_u64(unsigned long long *ptr, int64_t n, hipStream_t stream) {
@@ -523,11 +643,11 @@ hipError_t launch_zero_u64(unsigned long long *ptr, int64_t n, hipStream_t strea
     return hipGetLastError();
 }
 
-template <int MODE, bool XCORR>
+template <int MODE, bool XCORR, bool GENERAL, bool LDS_HIST>
 static hipError_t launch_xi_t(const XiConst &C, const ForestView &F, const PairView &P,
                               const GroupView &G, const XiOut &O, int grid, size_t lds,
                               hipStream_t stream) {
-    auto kern = xi_kernel<MODE, XCORR>;
+    auto kern = xi_kernel<MODE, XCORR, GENERAL, LDS_HIST>;
     hipError_t err = hipFuncSetAttribute((const void *)kern,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (err != hipSuccess) return err;
@@ -535,17 +655,30 @@ static hipError_t launch_xi_t(const XiConst &C, const ForestView &F, const PairV
     return hipGetLastError();
 }
 
-hipError_t launch_xi(int mode, bool xcorr, const XiConst &C, const ForestView &F,
-                     const PairView &P, const GroupView &G, const XiOut &O, int grid,
-                     size_t lds, hipStream_t stream) {
-    if (mode == MODE_STANDARD)
-        return xcorr ? launch_xi_t<MODE_STANDARD, true>(C, F, P, G, O, grid, lds, stream)
-                     : launch_xi_t<MODE_STANDARD, false>(C, F, P, G, O, grid, lds, stream);
+template <int MODE, bool XCORR, bool GENERAL>
+static hipError_t launch_xi_h(bool lds_hist, const XiConst &C, const ForestView &F,
+                              const PairView &P, const GroupView &G, const XiOut &O, int grid,
+                              size_t lds, hipStream_t stream) {
+    return lds_hist ? launch_xi_t<MODE, XCORR, GENERAL, true>(C, F, P, G, O, grid, lds, stream)
+                    : launch_xi_t<MODE, XCORR, GENERAL, false>(C, F, P, G, O, grid, lds, stream);
+}
+
+// general == false is only valid for MODE_STANDARD, auto-correlation, r_par_min == 0 and
+// no z / zerr / same-half-plate cuts (the host decides).
+hipError_t launch_xi(int mode, bool xcorr, bool general, bool lds_hist, const XiConst &C,
+                     const ForestView &F, const PairView &P, const GroupView &G,
+                     const XiOut &O, int grid, size_t lds, hipStream_t stream) {
+    if (mode == MODE_STANDARD) {
+        if (!general && !xcorr)
+            return launch_xi_h<MODE_STANDARD, false, false>(lds_hist, C, F, P, G, O, grid, lds, stream);
+        return xcorr ? launch_xi_h<MODE_STANDARD, true, true>(lds_hist, C, F, P, G, O, grid, lds, stream)
+                     : launch_xi_h<MODE_STANDARD, false, true>(lds_hist, C, F, P, G, O, grid, lds, stream);
+    }
     if (mode == MODE_RMU)
-        return xcorr ? launch_xi_t<MODE_RMU, true>(C, F, P, G, O, grid, lds, stream)
-                     : launch_xi_t<MODE_RMU, false>(C, F, P, G, O, grid, lds, stream);
-    return xcorr ? launch_xi_t<MODE_ANGULAR, true>(C, F, P, G, O, grid, lds, stream)
-                 : launch_xi_t<MODE_ANGULAR, false>(C, F, P, G, O, grid, lds, stream);
+        return xcorr ? launch_xi_h<MODE_RMU, true, true>(lds_hist, C, F, P, G, O, grid, lds, stream)
+                     : launch_xi_h<MODE_RMU, false, true>(lds_hist, C, F, P, G, O, grid, lds, stream);
+    return xcorr ? launch_xi_h<MODE_ANGULAR, true, true>(lds_hist, C, F, P, G, O, grid, lds, stream)
+                 : launch_xi_h<MODE_ANGULAR, false, true>(lds_hist, C, F, P, G, O, grid, lds, stream);
 }
 
 }  // namespace picca

@@ -335,5 +335,9 @@ def test_repeatable_counts(small_survey):
     params = params_from_globals(Globals())
     r1 = batch.run_xi(s.store, pairs, params)
     r2 = batch.run_xi(s.store, pairs, params)
+    # counts are exact; FP64 sums depend on the order of the atomic adds only at the
+    # last-bits level (the xi sums cancel, so compare against their magnitude)
     assert np.array_equal(r1.num_pairs, r2.num_pairs)
-    np.testing.assert_allclose(r1.xi, r2.xi, rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(r1.weights, r2.weights, rtol=1e-12)
+    scale = np.abs(r1.xi).max()
+    np.testing.assert_allclose(r1.xi, r2.xi, rtol=1e-12, atol=1e-13 * scale)
