@@ -9,13 +9,20 @@ iterations of the reference's inner loop (py/picca/cf.py:330-387) -- divided by 
 time of one step.
 
 Beside it:
-  * roofline of the pair kernel (FP64 vector ALU), from the kernel's own HIP-event time;
+  * roofline of the pair kernel (FP64 vector ALU): the reference algorithm's FP64
+    operations for this launch (4 per evaluated pixel pair, 17 more per accepted one) over
+    the kernel's duration, taken with HIP events on the stream the kernel runs on;
   * a CPU baseline: the FP64 oracle (a scalar C restatement of the reference loop) on a
     seeded random sample of the same forest pairs, on this host's cores;
   * parity on that sample: num_pairs mismatches against the oracle and the largest
     relative xi error.
 
     python bench.py [--steps K] [--warmup W] [--workload survey_100k] [--scale 1.0]
+                    [--copy-probe MIB]
+
+--copy-probe runs a plain device-to-device copy of MIB mebibytes before the steps, so that
+a hardware-counter profile of the same command holds a kernel of known traffic to scale
+FETCH_SIZE / WRITE_SIZE by.
 """
 import argparse
 import json
@@ -73,6 +80,27 @@ def cpu_cores():
         return os.cpu_count() or 1
 
 
+def measured_traffic(workload, scale):
+    """HBM bytes per pair-kernel launch measured with FETCH_SIZE + WRITE_SIZE (each scaled
+    by the same counters on a plain copy), as recorded beside the profiles."""
+    path = os.path.join(ROOT, "profiles", "xi_kernel_hbm_traffic.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return 0.0
+    entry = table.get("%s@%g" % (workload, scale))
+    return float(entry["traffic_bytes"]) if entry else 0.0
+
+
+def copy_probe(eng, mib):
+    """A plain device copy kernel over `mib` MiB (calibrates HBM byte counters)."""
+    nbytes = int(mib * (1 << 20)) // 16 * 16
+    ms = eng.diag_copy(nbytes)
+    log("copy probe: %d bytes read, %d bytes written, %.3f ms (%.0f GB/s)"
+        % (nbytes, nbytes, ms, 2 * nbytes / (ms / 1e3) / 1e9))
+
+
 def sample_pairs(pairs, store, target_evaluated, seed=1):
     """Seeded random subset of forest pairs holding about target_evaluated pixel pairs."""
     nz = store.nonzero_weight_counts()
@@ -90,7 +118,7 @@ def main():
     parser = argparse.ArgumentParser(description=__doc__,
                                      formatter_class=argparse.RawDescriptionHelpFormatter)
     parser.add_argument("--gpus", type=int, default=1)
-    parser.add_argument("--steps", type=int, default=3)
+    parser.add_argument("--steps", type=int, default=2)
     parser.add_argument("--warmup", type=int, default=1)
     parser.add_argument("--workload", default="survey_100k")
     parser.add_argument("--scale", type=float, default=1.0,
@@ -99,6 +127,11 @@ def main():
     parser.add_argument("--cpu-pairs", type=float, default=1.5e10,
                         help="evaluated pixel pairs in the CPU-baseline sample")
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    parser.add_argument("--copy-probe", type=float, default=0.0,
+                        help="MiB of a plain device copy to run before the steps")
+    parser.add_argument("--traffic-bytes", type=float, default=-1.0,
+                        help="HBM bytes per launch of the pair kernel (default: the "
+                             "counter-derived figure in profiles/ for this workload)")
     args = parser.parse_args()
 
     if args.gpus != 1:
@@ -134,6 +167,11 @@ def main():
     upload_s = time.time() - t2
     log("device %s: upload %.1f s" % (info["name"], upload_s))
 
+    if args.traffic_bytes < 0:
+        args.traffic_bytes = measured_traffic(args.workload, args.scale)
+    if args.copy_probe > 0:
+        copy_probe(eng, args.copy_probe)
+
     for w in range(args.warmup):
         rep = eng.xi_execute(params)
         log("warmup %d: kernel %.1f ms" % (w, rep.kernel_ms))
@@ -151,12 +189,21 @@ def main():
     value = evaluated / (ms_per_step / 1e3)
     rep = report.as_dict()
 
-    # roofline of the pair kernel: FP64 operations of the pairs it evaluated
-    flops = FLOPS_EVALUATED * rep["visited_pairs"] + FLOPS_ACCEPTED_EXTRA * rep["accepted_pairs"]
+    # roofline of the pair kernel: the reference loop's FP64 operations for this launch
+    # (every evaluated pair, pruned or not, and the accepted ones), over the kernel time
+    flops = FLOPS_EVALUATED * evaluated + FLOPS_ACCEPTED_EXTRA * rep["accepted_pairs"]
     achieved = flops / (kernel_ms / 1e3) / 1e12
+    # the operations the kernel itself executes: it walks only the pixel pairs left after
+    # the r_par / r_trans window pruning
+    executed = FLOPS_EVALUATED * rep["visited_pairs"] + FLOPS_ACCEPTED_EXTRA * rep["accepted_pairs"]
     roofline = {"bound": "valu", "achieved": achieved, "peak": FP64_VALU_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP64_VALU_PEAK_TFLOPS, "traffic": None,
-                "flops_per_launch": flops, "kernel_ms": kernel_ms}
+                "unit": "TFLOP/s", "frac": achieved / FP64_VALU_PEAK_TFLOPS,
+                "traffic": args.traffic_bytes if args.traffic_bytes > 0 else None,
+                "flops_per_launch": flops,
+                "executed_tflops": executed / (kernel_ms / 1e3) / 1e12,
+                "kernel": "picca::xi_kernel<0,false,false,true>", "kernel_ms": kernel_ms,
+                "hbm_gbs_at_traffic": (args.traffic_bytes / (kernel_ms / 1e3) / 1e9
+                                       if args.traffic_bytes > 0 else None)}
 
     result = {
         "metric": METRIC,

@@ -146,6 +146,14 @@ int picca_dmat_run(picca_ctx *ctx, const picca_cf_params *params, int64_t num_pa
                    double *weight_eff, int64_t *stats);
 
 /*
+ * Diagnostics (no counterpart in picca): runs a plain device-to-device copy kernel over
+ * `bytes` bytes (a multiple of 16) between two buffers of the context and returns its
+ * device time.  Used to calibrate hardware byte counters (FETCH_SIZE / WRITE_SIZE) against
+ * a kernel of known traffic.
+ */
+int picca_diag_copy(picca_ctx *ctx, int64_t bytes, double *kernel_ms);
+
+/*
  * Neighbour candidates for fill_neighs (py/picca/cf.py:82-135), host code.  Replaces the
  * healpy.query_disc(inclusive=True) superset (py/picca/cf.py:93-98): returns, per source,
  * the targets (ascending index) with x2*x1 + y2*y1 + z2*z1 >= cos_min or inside the

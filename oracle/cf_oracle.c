@@ -203,6 +203,7 @@ static int cmp_i32(const void *x, const void *y) {
  * Returns 0, or -1 when a selected pair has a negative data bin (the reference raises
  * IndexError there, py/picca/cf.py:855-856), or -2 on allocation failure.
  * stats[0] in-range pairs of the main pass, stats[1] selected pairs,
+ * stats[3] selected pairs whose data bin is >= np*nt (left out),
  * stats[2] in-range pairs beyond the count-pass size (the reference writes those
  * model bins past the end of all_model_bins; np.unique then sees only the first
  * count-pass-many, which is what this function reproduces).
@@ -352,7 +353,13 @@ int oracle_dmat_pair(const oracle_params *p,
                 r_trans_eff[model_bins] += weights12 * r_trans;
                 z_eff[model_bins] += weights12 * z;
                 weight_eff[model_bins] += weights12;
-                weights_dmat[bins] += weights12;
+                /* numba would write out of bounds for bins >= np*nt (bin rounding at the
+                 * range limit): such pairs are counted and left out, as in the fill */
+                if (bins < (int64_t)p->num_bins_r_par * p->num_bins_r_trans) {
+                    if (bins >= 0) weights_dmat[bins] += weights12;
+                } else {
+                    stats[3] += 1;
+                }
             }
             eta1[i + n1 * model_bins] += zfac * weights2[j] / sum_weights2;
             eta2[j + n2 * model_bins] += zfac * weights1[i] / sum_weights1;

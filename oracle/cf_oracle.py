@@ -181,6 +181,35 @@ def xi_rows(params, store, pair_a, pair_b, pair_ang, pair_flags, pair_row, num_r
     return out, stats
 
 
+def dmat_rows(params, store, pair_a, pair_b, pair_ang, pair_flags):
+    """compute_dmat_forest_pairs_fast over an explicit list of kept pairs of a store, in
+    list order, accumulated into one set of outputs.
+
+    Returns (weights_dmat[nb], dmat[nb, nm], r_par_eff, r_trans_eff, z_eff, weight_eff) and
+    stats[4].
+    """
+    nb = params.num_bins_r_par * params.num_bins_r_trans
+    nm = params.num_model_bins_r_par * params.num_model_bins_r_trans
+    out = [np.zeros(nb), np.zeros(nb * nm)] + [np.zeros(nm) for _ in range(4)]
+    stats = np.zeros(4, dtype=np.int64)
+    a = np.ascontiguousarray(pair_a, dtype=np.int32)
+    b = np.ascontiguousarray(pair_b, dtype=np.int32)
+    ang = np.ascontiguousarray(pair_ang, dtype=np.float64)
+    shp = np.ascontiguousarray(np.asarray(pair_flags) & 1, dtype=np.uint8)
+    order = np.ascontiguousarray(store.order, dtype=np.int32)
+    status = lib().oracle_dmat_rows(ctypes.byref(params), _p(store.offsets),
+                                    _p(store.log_lambda), _p(store.z), _p(store.r_comov),
+                                    _p(store.dist_m), _p(store.weights), _p(store.z_qso),
+                                    _p(order), len(a), _p(a), _p(b), _p(ang), _p(shp),
+                                    *[_p(x) for x in out], _p(stats))
+    if status == -1:
+        raise IndexError("negative bin index")
+    if status != 0:
+        raise MemoryError("oracle_dmat_rows failed")
+    out[1] = out[1].reshape(nb, nm)
+    return out, stats
+
+
 def dmat_pair(params, ll1, ll2, r1, r2, d1, d2, z1, z2, w1, w2, zq1, zq2, ang, out,
               same_half_plate, order1, order2, stats):
     arrays = [_f(x) for x in (ll1, ll2, r1, r2, d1, d2, z1, z2, w1, w2)]
@@ -284,7 +313,7 @@ def compute_dmat(params, data, healpixs, neighbours, reject):
     weights_dmat = np.zeros(nb)
     dmat = np.zeros(nb * nm)
     eff = [np.zeros(nm) for _ in range(4)]
-    stats = np.zeros(3, dtype=np.int64)
+    stats = np.zeros(4, dtype=np.int64)
     num_pairs = 0
     num_pairs_used = 0
     for hp in healpixs:

@@ -32,7 +32,7 @@ def hipcc():
 
 
 def build(verbose=False):
-    sources = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    sources = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIBRARY + ".tmp"
     cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp] + sources + ["-lpthread"]
     if verbose:

@@ -387,7 +387,7 @@ def compute_dmat(healpixs):
     (weights_dmat, dmat, r_par_eff, r_trans_eff, z_eff, weight_eff,
      stats) = eng.dmat(params, pair_a, pair_b, pair_ang, flags)
     last_report = {"in_range_pairs": int(stats[0]), "selected_pairs": int(stats[1]),
-                   "beyond_count_pass": int(stats[2])}
+                   "beyond_count_pass": int(stats[2]), "overflow_pairs": int(stats[3])}
     return (weights_dmat, dmat, r_par_eff, r_trans_eff, z_eff, weight_eff, num_pairs,
             num_pairs_used)
 

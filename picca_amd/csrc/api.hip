@@ -122,6 +122,22 @@ struct picca_ctx {
     DeviceBuffer<int64_t> gbegin;
     DeviceBuffer<int32_t> grow;
 
+    // distortion-matrix inputs (stores given with log_lambda): per-forest sums of
+    // compute_dmat_forest_pairs_fast (py/picca/cf.py:577-594) and the pixel redshifts
+    std::vector<double> h_z;
+    DeviceBuffer<double> dll, sum_w, sw_dll, zfa, zfb;
+
+    // dmat batches
+    DeviceBuffer<int32_t> dpa, dpb;
+    DeviceBuffer<double> dang, dcs;
+    DeviceBuffer<uint8_t> dflags;
+    DeviceBuffer<double> ws_S, ws_eta;
+    DeviceBuffer<uint32_t> ws_U;
+    DeviceBuffer<int32_t> ws_band, ws_rng, ws_cnt;
+    DeviceBuffer<int64_t> ws_pix;
+    DeviceBuffer<unsigned long long> ws_touched, dstats;
+    DeviceBuffer<double> o_wdm, o_dmat, o_eff;
+
     // xi results
     int64_t out_nb = 0;
     int32_t out_rows = 0;
@@ -323,7 +339,37 @@ int picca_set_forests(picca_ctx *ctx, int64_t num_forests, const int64_t *offset
     PICCA_HIP_CHECK(ctx->zq.upload(z_qso, (size_t)num_forests, ctx->stream));
     PICCA_HIP_CHECK(ctx->mono.upload(mono.data(), (size_t)num_forests, ctx->stream));
     ctx->has_log_lambda = log_lambda != nullptr;
-    if (log_lambda) PICCA_HIP_CHECK(ctx->ll.upload(log_lambda, (size_t)npix, ctx->stream));
+    if (log_lambda) {
+        PICCA_HIP_CHECK(ctx->ll.upload(log_lambda, (size_t)npix, ctx->stream));
+        // per forest, sequential sums in pixel order as the reference's loops:
+        // sum w, weighted mean log_lambda, dll = log_lambda - mean, sum w dll^2
+        std::vector<double> dll((size_t)std::max<int64_t>(npix, 1));
+        std::vector<double> sum_w((size_t)std::max<int64_t>(num_forests, 1));
+        std::vector<double> sw_dll((size_t)std::max<int64_t>(num_forests, 1));
+        parallel_for(num_forests, [&](int64_t lo, int64_t hi) {
+            for (int64_t k = lo; k < hi; ++k) {
+                const int64_t o = offsets[k], e = offsets[k + 1];
+                double sw = 0, acc = 0;
+                for (int64_t q = o; q < e; ++q) sw += weights[q];
+                for (int64_t q = o; q < e; ++q) acc += log_lambda[q] * weights[q];
+                const double mean = acc / sw;
+                double s2 = 0;
+                for (int64_t q = o; q < e; ++q) {
+                    dll[q] = log_lambda[q] - mean;
+                    s2 += weights[q] * (dll[q] * dll[q]);
+                }
+                sum_w[k] = sw;
+                sw_dll[k] = s2;
+            }
+        });
+        PICCA_HIP_CHECK(ctx->dll.upload_padded(dll.data(), (size_t)npix, pad, ctx->stream));
+        PICCA_HIP_CHECK(ctx->sum_w.upload(sum_w.data(), (size_t)num_forests, ctx->stream));
+        PICCA_HIP_CHECK(ctx->sw_dll.upload(sw_dll.data(), (size_t)num_forests, ctx->stream));
+        PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx->h_z.assign(z, z + npix);
+    } else {
+        ctx->h_z.clear();
+    }
     ctx->has_order = order != nullptr;
     if (order) PICCA_HIP_CHECK(ctx->order.upload(order, (size_t)num_forests, ctx->stream));
     PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -509,12 +555,227 @@ int picca_xi_download(picca_ctx *ctx, double *weights, double *xi, double *r_par
     return PICCA_OK;
 }
 
+int picca_diag_copy(picca_ctx *ctx, int64_t bytes, double *kernel_ms) {
+    if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
+    if (bytes <= 0 || bytes % 16 != 0)
+        return fail(PICCA_E_VALUE, "bytes must be a positive multiple of 16");
+    PICCA_HIP_CHECK(hipSetDevice(ctx->device));
+    DeviceBuffer<unsigned long long> src, dst;
+    const int64_t words = bytes / 8;
+    PICCA_HIP_CHECK(src.reserve((size_t)words));
+    PICCA_HIP_CHECK(dst.reserve((size_t)words));
+    PICCA_HIP_CHECK(picca::launch_zero_u64(src.ptr, words, ctx->stream));
+    PICCA_HIP_CHECK(hipEventRecord(ctx->ev_start, ctx->stream));
+    PICCA_HIP_CHECK(picca::launch_copy16(src.ptr, dst.ptr, bytes / 16, ctx->stream));
+    PICCA_HIP_CHECK(hipEventRecord(ctx->ev_stop, ctx->stream));
+    PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    PICCA_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    if (kernel_ms) *kernel_ms = (double)ms;
+    return PICCA_OK;
+}
+
 int picca_dmat_run(picca_ctx *ctx, const picca_cf_params *params, int64_t num_pairs,
                    const int32_t *pair_a, const int32_t *pair_b, const double *pair_ang,
                    const uint8_t *pair_flags, double *weights_dmat, double *dmat,
                    double *r_par_eff, double *r_trans_eff, double *z_eff,
                    double *weight_eff, int64_t *stats) {
-    return fail(PICCA_E_RUNTIME, "picca_dmat_run is not available in this build");
+    if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
+    picca::XiConst X;
+    int mode = 0;
+    int status = build_xi_const(params, X, mode);
+    if (status != PICCA_OK) return status;
+    if (params->ang_correlation)
+        return fail(PICCA_E_VALUE, "the distortion matrix has no angular mode");
+    if (params->num_model_bins_r_par <= 0 || params->num_model_bins_r_trans <= 0)
+        return fail(PICCA_E_VALUE, "num_model_bins_r_par and num_model_bins_r_trans must be "
+                                   "positive");
+    if (!ctx->has_log_lambda || !ctx->has_order)
+        return fail(PICCA_E_VALUE, "the distortion matrix needs a forest store with "
+                                   "log_lambda and order");
+    if (num_pairs < 0) return fail(PICCA_E_VALUE, "negative size");
+    if (num_pairs > 0 && (!pair_a || !pair_b || !pair_ang || !pair_flags))
+        return fail(PICCA_E_VALUE, "pair arrays must not be NULL");
+    if (!weights_dmat || !dmat || !r_par_eff || !r_trans_eff || !z_eff || !weight_eff)
+        return fail(PICCA_E_VALUE, "output arrays must not be NULL");
+    const int64_t nf = ctx->num_forests;
+    for (int64_t k = 0; k < num_pairs; ++k) {
+        if (pair_a[k] < 0 || pair_a[k] >= nf || pair_b[k] < 0 || pair_b[k] >= nf)
+            return fail(PICCA_E_VALUE, "pair index out of range of the forest store");
+        if (!(pair_ang[k] >= 0.0) || !(pair_ang[k] <= M_PI))
+            return fail(PICCA_E_VALUE, "pair angle outside [0, pi]");
+    }
+
+    picca::DmatConst C;
+    std::memset(&C, 0, sizeof(C));
+    C.np = X.np;
+    C.nt = X.nt;
+    C.nb = X.nb;
+    C.npm = params->num_model_bins_r_par;
+    C.ntm = params->num_model_bins_r_trans;
+    if ((int64_t)C.npm * C.ntm > (1 << 26) || (int64_t)C.nb * ((int64_t)C.npm * C.ntm) > (1ll << 32))
+        return fail(PICCA_E_VALUE, "too many model bins");
+    C.nm = C.npm * C.ntm;
+    C.flags = X.flags & (picca::XI_HAS_ZMIN | picca::XI_HAS_ZMAX | picca::XI_HAS_ZERR |
+                         picca::XI_REMOVE_SHP);
+    if (params->redshift_evolution) C.flags |= picca::DMAT_REDSHIFT_EVOL;
+    if (params->x_correlation) C.flags |= picca::DMAT_XCORR;
+    if (params->rmu_binning) C.flags |= picca::DMAT_RMU;
+    C.words_u = (C.nm + 31) / 32;
+    C.rpmin = X.rpmin;
+    C.rpmax = X.rpmax;
+    C.rtmax = X.rtmax;
+    C.span_p = X.span_p;
+    C.width_p = X.width_p;
+    C.zmin = X.zmin;
+    C.zmax = X.zmax;
+    C.zerr_ang = X.zerr_ang;
+    C.zerr_kms = X.zerr_kms;
+    C.np_d = C.np;
+    C.nt_d = C.nt;
+    C.npm_d = C.npm;
+    C.ntm_d = C.ntm;
+    // LDS of the pass kernel: U bitmap, row ranges, then bands of k r_trans rows
+    // (4 sums per data bin, 8 per model bin)
+    const size_t fixed = ((size_t)C.words_u * 4 + (size_t)(4 * C.nt + 4 * C.ntm) * 4 + 15) &
+                         ~(size_t)15;
+    const size_t lds_budget = std::min<size_t>(120 * 1024, (size_t)ctx->lds_limit);
+    const size_t per_row = 32 * (size_t)C.np + 64 * (size_t)C.npm;
+    if (fixed + per_row > lds_budget)
+        return fail(PICCA_E_VALUE, "too many bins for the distortion-matrix pass kernel");
+    const int k_rows = (int)std::min<size_t>((lds_budget - fixed) / per_row,
+                                             (size_t)std::max(C.nt, C.ntm));
+    C.tb_max = std::min(C.nt, k_rows);
+    C.ub_max = std::min(C.ntm, k_rows);
+    const size_t pass_lds = fixed + 32 * (size_t)C.np * C.tb_max + 64 * (size_t)C.npm * C.ub_max;
+
+    PICCA_HIP_CHECK(hipSetDevice(ctx->device));
+    PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const int64_t npix = ctx->num_pixels;
+    const size_t pad = picca::PIXEL_PAD;
+    {   // redshift-evolution factors of each pixel (py/picca/cf.py:680-685)
+        std::vector<double> fa((size_t)std::max<int64_t>(npix, 1), 1.0);
+        std::vector<double> fb((size_t)std::max<int64_t>(npix, 1), 1.0);
+        if (params->redshift_evolution) {
+            const double zr = 1 + params->z_ref;
+            const double ea = params->alpha - 1, eb = params->alpha2 - 1;
+            parallel_for(npix, [&](int64_t lo, int64_t hi) {
+                for (int64_t k = lo; k < hi; ++k) {
+                    fa[k] = std::pow((1 + ctx->h_z[k]) / zr, ea);
+                    fb[k] = std::pow((1 + ctx->h_z[k]) / zr, eb);
+                }
+            });
+        }
+        PICCA_HIP_CHECK(ctx->zfa.upload_padded(fa.data(), (size_t)npix, pad, ctx->stream));
+        PICCA_HIP_CHECK(ctx->zfb.upload_padded(fb.data(), (size_t)npix, pad, ctx->stream));
+    }
+    std::vector<double> cs((size_t)std::max<int64_t>(2 * num_pairs, 1));
+    for (int64_t k = 0; k < num_pairs; ++k) {
+        cs[2 * k] = std::cos(pair_ang[k] / 2);
+        cs[2 * k + 1] = std::sin(pair_ang[k] / 2);
+    }
+    PICCA_HIP_CHECK(ctx->dpa.upload(pair_a, (size_t)num_pairs, ctx->stream));
+    PICCA_HIP_CHECK(ctx->dpb.upload(pair_b, (size_t)num_pairs, ctx->stream));
+    PICCA_HIP_CHECK(ctx->dang.upload(pair_ang, (size_t)num_pairs, ctx->stream));
+    PICCA_HIP_CHECK(ctx->dcs.upload(cs.data(), (size_t)(2 * num_pairs), ctx->stream));
+    PICCA_HIP_CHECK(ctx->dflags.upload(pair_flags, (size_t)num_pairs, ctx->stream));
+
+    // outputs, accumulated over all batches on the device
+    const int64_t nb = C.nb, nm = C.nm;
+    PICCA_HIP_CHECK(ctx->o_wdm.reserve((size_t)nb));
+    PICCA_HIP_CHECK(ctx->o_dmat.reserve((size_t)(nb * nm)));
+    PICCA_HIP_CHECK(ctx->o_eff.reserve((size_t)(4 * nm)));
+    PICCA_HIP_CHECK(ctx->dstats.reserve(picca::DMAT_NSTATS));
+    PICCA_HIP_CHECK(picca::launch_zero_u64((unsigned long long *)ctx->o_wdm.ptr, nb, ctx->stream));
+    PICCA_HIP_CHECK(picca::launch_zero_u64((unsigned long long *)ctx->o_dmat.ptr, nb * nm,
+                                           ctx->stream));
+    PICCA_HIP_CHECK(picca::launch_zero_u64((unsigned long long *)ctx->o_eff.ptr, 4 * nm,
+                                           ctx->stream));
+    PICCA_HIP_CHECK(picca::launch_zero_u64(ctx->dstats.ptr, picca::DMAT_NSTATS, ctx->stream));
+
+    // workspace per slot
+    const size_t slot_bytes = 8 * (4 * (size_t)C.nt * C.np + 4 * (size_t)C.ntm * C.npm) +
+                              4 * (size_t)C.words_u + 16 + 16 * (size_t)(C.nt + C.ntm);
+    const size_t ws_budget = (size_t)1536 << 20;
+    const int64_t cap = std::max<int64_t>(
+        1, std::min<int64_t>({num_pairs, (int64_t)(ws_budget / slot_bytes), (int64_t)1 << 20}));
+    const int64_t touched_words = ((int64_t)C.nt * C.ntm + 63) / 64;
+    PICCA_HIP_CHECK(ctx->ws_S.reserve((size_t)cap * 4 * C.nt * C.np));
+    PICCA_HIP_CHECK(ctx->ws_eta.reserve((size_t)cap * 4 * C.ntm * C.npm));
+    PICCA_HIP_CHECK(ctx->ws_U.reserve((size_t)cap * C.words_u));
+    PICCA_HIP_CHECK(ctx->ws_band.reserve((size_t)cap * 4));
+    PICCA_HIP_CHECK(ctx->ws_rng.reserve((size_t)cap * 4 * (C.nt + C.ntm)));
+    PICCA_HIP_CHECK(ctx->ws_pix.reserve((size_t)cap));
+    PICCA_HIP_CHECK(ctx->ws_touched.reserve((size_t)touched_words));
+
+    picca::DmatForests F{ctx->r.ptr,   ctx->d.ptr,     ctx->z.ptr,     ctx->w.ptr,   ctx->dll.ptr,
+                         ctx->zfa.ptr, ctx->zfb.ptr,   ctx->off.ptr,   ctx->zq.ptr,  ctx->sum_w.ptr,
+                         ctx->sw_dll.ptr, ctx->order.ptr, ctx->mono.ptr};
+    picca::DmatOut O{ctx->o_wdm.ptr, ctx->o_dmat.ptr, ctx->o_eff.ptr, ctx->o_eff.ptr + nm,
+                     ctx->o_eff.ptr + 2 * nm, ctx->o_eff.ptr + 3 * nm};
+    std::vector<int64_t> pix_off;
+    for (int64_t p0 = 0; p0 < num_pairs; p0 += cap) {
+        const int64_t count = std::min(cap, num_pairs - p0);
+        pix_off.assign((size_t)count, 0);
+        int64_t total = 0;
+        for (int64_t k = 0; k < count; ++k) {
+            pix_off[k] = total;
+            const int32_t a = pair_a[p0 + k];
+            total += ctx->h_offsets[a + 1] - ctx->h_offsets[a];
+        }
+        PICCA_HIP_CHECK(ctx->ws_pix.upload(pix_off.data(), (size_t)count, ctx->stream));
+        PICCA_HIP_CHECK(ctx->ws_cnt.reserve((size_t)std::max<int64_t>(total, 1)));
+        PICCA_HIP_CHECK(picca::launch_zero_u64(ctx->ws_touched.ptr, touched_words, ctx->stream));
+        picca::DmatPairs P{ctx->dpa.ptr + p0, ctx->dpb.ptr + p0, ctx->dang.ptr + p0,
+                           ctx->dcs.ptr + 2 * p0, ctx->dflags.ptr + p0, (int32_t)count};
+        int32_t *rng = ctx->ws_rng.ptr;
+        picca::DmatWork W{ctx->ws_S.ptr,
+                          ctx->ws_eta.ptr,
+                          ctx->ws_U.ptr,
+                          ctx->ws_band.ptr,
+                          rng,
+                          rng + (size_t)count * 2 * C.nt,
+                          rng + (size_t)count * (2 * C.nt + 2 * C.ntm),
+                          rng + (size_t)count * (4 * C.nt + 2 * C.ntm),
+                          ctx->ws_cnt.ptr,
+                          ctx->ws_pix.ptr,
+                          (uint32_t *)ctx->ws_touched.ptr,
+                          ctx->dstats.ptr};
+        PICCA_HIP_CHECK(picca::launch_dmat_pass(C, F, P, W, O, pass_lds, ctx->stream));
+        const int groups = (int)std::max<int64_t>(1, std::min<int64_t>(32, count / 16));
+        PICCA_HIP_CHECK(picca::launch_dmat_fill(C, F, P, W, O, groups, ctx->stream));
+    }
+
+    std::vector<double> h_wdm((size_t)nb), h_dmat((size_t)(nb * nm)), h_eff((size_t)(4 * nm));
+    unsigned long long h_stats[picca::DMAT_NSTATS];
+    PICCA_HIP_CHECK(hipMemcpyAsync(h_wdm.data(), ctx->o_wdm.ptr, sizeof(double) * nb,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    PICCA_HIP_CHECK(hipMemcpyAsync(h_dmat.data(), ctx->o_dmat.ptr, sizeof(double) * nb * nm,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    PICCA_HIP_CHECK(hipMemcpyAsync(h_eff.data(), ctx->o_eff.ptr, sizeof(double) * 4 * nm,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    PICCA_HIP_CHECK(hipMemcpyAsync(h_stats, ctx->dstats.ptr, sizeof(h_stats),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (h_stats[picca::DMAT_STAT_NEGATIVE] > 0)
+        return fail(PICCA_E_VALUE, "a selected pair has a negative data bin index");
+    for (int64_t k = 0; k < nb; ++k) weights_dmat[k] += h_wdm[k];
+    parallel_for(nb * nm, [&](int64_t lo, int64_t hi) {
+        for (int64_t k = lo; k < hi; ++k) dmat[k] += h_dmat[k];
+    });
+    for (int64_t k = 0; k < nm; ++k) {
+        r_par_eff[k] += h_eff[k];
+        r_trans_eff[k] += h_eff[nm + k];
+        z_eff[k] += h_eff[2 * nm + k];
+        weight_eff[k] += h_eff[3 * nm + k];
+    }
+    if (stats) {
+        stats[0] = (int64_t)h_stats[picca::DMAT_STAT_IN_RANGE];
+        stats[1] = (int64_t)h_stats[picca::DMAT_STAT_SELECTED];
+        stats[2] = (int64_t)h_stats[picca::DMAT_STAT_BEYOND];
+        stats[3] = (int64_t)h_stats[picca::DMAT_STAT_OVERFLOW];
+    }
+    return PICCA_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,865 @@
+// Distortion matrix on gfx950 (MI355X): compute_dmat_forest_pairs_fast
+// (py/picca/cf.py:520-887) for a batch of kept forest pairs.
+//
+// The reference's fill loop (py/picca/cf.py:846-887) adds, for every selected pixel pair
+// (i, j) in data bin A and every model bin k of the pair's unique model bins U,
+//     w1 w2 (eta5 + eta6 dll2 + eta7 dll1 + eta8 dll1 dll2 - eta1[i] - eta2[j]
+//            - eta3[i] dll2 - eta4[j] dll1)[k]
+// to dmat[A, k], plus w1 w2 zfac to dmat[A, M(i, j)].  Grouping the sum by pixel gives
+//     dmat[A, k] += S0[A] eta5[k] + Sj[A] eta6[k] + Si[A] eta7[k] + Sij[A] eta8[k]
+//                 - sum_i (P[A, i] eta1[i, k] + P'[A, i] eta3[i, k])
+//                 - sum_j (Q[A, j] eta2[j, k] + Q'[A, j] eta4[j, k])
+// with P[A, i] = sum_j w1 w2, P' = sum_j w1 w2 dll2 over the selected pairs of pixel i in
+// bin A (Q, Q' the same over i for pixel j) and S = the sums of P, P'.  eta1[i, .] and
+// P[., i] are non-zero only in the one or two r_trans rows that pixel i's pairs reach, so
+// the products are dense in r_par and banded in r_trans.
+//
+// Two kernels per batch:
+//   dmat_pass_kernel  one workgroup per forest pair, lane per pixel i: selection, bins,
+//                     eta5..eta8 and S in LDS bands, the effective-coordinate sums and
+//                     weights_dmat, the unique model bins U (with the reference's
+//                     count-pass truncation), and for each r_trans row the range of
+//                     pixels that reach it;
+//   dmat_fill_kernel  one workgroup per (data r_trans row t, model r_trans row u, slot
+//                     group): an np x npm block of dmat held in registers, accumulating
+//                     the S x eta rank-4 term, and per pixel i (then j) the vectors P, P',
+//                     eta1, eta3 (Q, Q', eta2, eta4) built by walking that pixel's pairs,
+//                     followed by their outer products; one atomic add per block entry.
+// All arithmetic is FP64; bins follow the reference's divide-then-multiply.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <limits.h>
+
+#include "device_util.h"
+#include "engine.h"
+
+namespace picca {
+
+namespace {
+
+struct DmatPair {
+    int a, b, la, lb;
+    int64_t oa, ob;
+    double ang, cos_half, sin_half;
+    double zq1, zq2, sum_w1, sum_w2, sw1, sw2;
+    bool shp_cut;   // remove_same_half_plate_close_pairs and the pair is on one half plate
+    bool zerr_on;   // zerr cut active for this pair (ang < zerr_cut_deg)
+    bool order1, order2;
+    bool prune;     // standard binning, both forests monotone, cos(ang/2) > 0
+};
+
+__device__ DmatPair load_pair(const DmatConst &C, const DmatForests &F, const DmatPairs &P,
+                              int s) {
+    DmatPair q;
+    q.a = uniform_int(P.a[s]);
+    q.b = uniform_int(P.b[s]);
+    q.oa = uniform_i64(F.off[q.a]);
+    q.la = (int)(uniform_i64(F.off[q.a + 1]) - q.oa);
+    q.ob = uniform_i64(F.off[q.b]);
+    q.lb = (int)(uniform_i64(F.off[q.b + 1]) - q.ob);
+    q.ang = uniform_f64(P.ang[s]);
+    q.cos_half = uniform_f64(P.cs[2 * s]);
+    q.sin_half = uniform_f64(P.cs[2 * s + 1]);
+    q.zq1 = uniform_f64(F.zq[q.a]);
+    q.zq2 = uniform_f64(F.zq[q.b]);
+    q.sum_w1 = uniform_f64(F.sum_w[q.a]);
+    q.sum_w2 = uniform_f64(F.sum_w[q.b]);
+    q.sw1 = uniform_f64(F.sw_dll[q.a]);
+    q.sw2 = uniform_f64(F.sw_dll[q.b]);
+    q.shp_cut = (C.flags & XI_REMOVE_SHP) &&
+                (uniform_int(P.flags[s]) & PAIR_SAME_HALF_PLATE) != 0;
+    q.zerr_on = (C.flags & XI_HAS_ZERR) && (q.ang < C.zerr_ang);
+    q.order1 = uniform_int(F.order[q.a]) == 1;
+    q.order2 = uniform_int(F.order[q.b]) == 1;
+    q.prune = !(C.flags & DMAT_RMU) && q.cos_half > 0 &&
+              (uniform_int(F.mono[q.a]) & 3) == 3 && (uniform_int(F.mono[q.b]) & 3) == 3;
+    return q;
+}
+
+// zerr test of py/picca/cf.py:628-635 / :647-654: true when the pixel is cut
+__device__ __forceinline__ bool zerr_cut(const DmatConst &C, double z, double z_qso) {
+    double z_qf = 0.5 * (z + z_qso);
+    double dv_kms = fabs(z - z_qso) / (1 + z_qf);
+    dv_kms *= SPEED_LIGHT_KMS;
+    return dv_kms < C.zerr_kms;
+}
+
+// Separations, range test and data/model bins of one pixel pair with the reference's
+// arithmetic (py/picca/cf.py:656-700).  A and M are the flat bin indices.
+__device__ __forceinline__ bool pair_bins(const DmatConst &C, double r1, double d1, double r2,
+                                          double d2, double cos_half, double sin_half,
+                                          double &r_par, double &r_trans, int &A, int &M) {
+    r_par = (r1 - r2) * cos_half;
+    r_trans = (d1 + d2) * sin_half;
+    if (C.flags & DMAT_RMU) {
+        r_trans = sqrt(r_trans * r_trans + r_par * r_par);
+        r_par /= r_trans;
+    }
+    if (!(C.flags & DMAT_XCORR)) r_par = fabs(r_par);
+    if (r_par >= C.rpmax || r_trans >= C.rtmax || r_par < C.rpmin) return false;
+    const double x = (r_par - C.rpmin) / C.span_p;
+    const double y = r_trans / C.rtmax;
+    const double bp = floor(x * C.np_d);
+    const double bt = floor(y * C.nt_d);
+    A = (int)(bt + C.nt_d * bp);
+    const double mp = floor(x * C.npm_d);
+    const double mt = floor(y * C.ntm_d);
+    M = (int)(mt + C.ntm_d * mp);
+    return true;
+}
+
+// Pair-redshift, zerr (pixel j) and same-half-plate parts of the selection
+// (py/picca/cf.py:640-676); the zerr part of pixel i is passed in.
+__device__ __forceinline__ bool pair_selected(const DmatConst &C, const DmatPair &q, bool i_sel,
+                                              bool j_zerr_cut, double z, double r_par) {
+    bool sel = i_sel && !j_zerr_cut;
+    if ((C.flags & XI_HAS_ZMIN) && z < C.zmin) sel = false;
+    if ((C.flags & XI_HAS_ZMAX) && z > C.zmax) sel = false;
+    if (q.shp_cut && fabs(r_par) < C.width_p) sel = false;
+    return sel;
+}
+
+// Window [lo, hi) of the other forest's pixels outside which every pair with this wave's
+// coordinates fails the range test.  `r_lo`/`r_hi` bound the partner's r_comov, `d_lim`
+// its dist_m; the slack is far above the rounding of the separations.
+__device__ void window(const double *__restrict__ Or, const double *__restrict__ Od, int n,
+                       double r_lo, double r_hi, double d_lim, bool use_d, int lane, int &lo,
+                       int &hi) {
+    r_lo -= 1e-6 + 1e-12 * fabs(r_lo);
+    r_hi += 1e-6 + 1e-12 * fabs(r_hi);
+    lo = wave_search<true>(Or, n, r_lo, lane);
+    hi = wave_search<false>(Or, n, r_hi, lane);
+    if (use_d) {
+        d_lim += 1e-6 + 1e-12 * fabs(d_lim);
+        const int h = wave_search<false>(Od, n, d_lim, lane);
+        if (h < hi) hi = h;
+    }
+    lo = uniform_int(lo);
+    hi = uniform_int(hi);
+}
+
+// pixels of delta2 that pixels of delta1 with r_comov in [r_min, r_max] and dist_m >= d_min
+// can pair with
+__device__ void window_b(const DmatConst &C, const DmatPair &q, const DmatForests &F,
+                         double r_min, double r_max, double d_min, int lane, int &lo,
+                         int &hi) {
+    lo = 0;
+    hi = q.lb;
+    if (!q.prune) return;
+    if (!(r_min <= r_max)) {
+        hi = 0;
+        return;
+    }
+    const double r_lo = r_min - C.rpmax / q.cos_half;
+    const double r_hi = (C.flags & DMAT_XCORR) ? r_max - C.rpmin / q.cos_half
+                                               : r_max + C.rpmax / q.cos_half;
+    window(F.r + q.ob, F.d + q.ob, q.lb, r_lo, r_hi, C.rtmax / q.sin_half - d_min,
+           q.sin_half > 0, lane, lo, hi);
+}
+
+// pixels of delta1 that pixel j of delta2 (r2, d2) can pair with
+__device__ void window_a(const DmatConst &C, const DmatPair &q, const DmatForests &F, double r2,
+                         double d2, int lane, int &lo, int &hi) {
+    lo = 0;
+    hi = q.la;
+    if (!q.prune) return;
+    const double r_lo = (C.flags & DMAT_XCORR) ? r2 + C.rpmin / q.cos_half
+                                               : r2 - C.rpmax / q.cos_half;
+    const double r_hi = r2 + C.rpmax / q.cos_half;
+    window(F.r + q.oa, F.d + q.oa, q.la, r_lo, r_hi, C.rtmax / q.sin_half - d2,
+           q.sin_half > 0, lane, lo, hi);
+}
+
+__device__ __forceinline__ void lds_add(double *p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Records, for a run of consecutive partner pixels of lane pixel `self` that share row
+// `row`, the pixel ranges of both forests.
+__device__ __forceinline__ void note_run(int32_t *self_rng, int32_t *other_rng, int row,
+                                         int self, int first, int last) {
+    atomicMin(&self_rng[2 * row], self);
+    atomicMax(&self_rng[2 * row + 1], self);
+    atomicMin(&other_rng[2 * row], first);
+    atomicMax(&other_rng[2 * row + 1], last);
+}
+
+struct RowRun {
+    int row, first, last;
+};
+
+__device__ __forceinline__ void row_step(RowRun &run, int row, int j, int32_t *self_rng,
+                                         int32_t *other_rng, int self) {
+    if (row != run.row) {
+        if (run.row >= 0) note_run(self_rng, other_rng, run.row, self, run.first, run.last);
+        run.row = row;
+        run.first = j;
+    }
+    run.last = j;
+}
+
+// Sums of one lane over consecutive pairs with the same bin, flushed to an LDS band.
+struct RunA {
+    int key;
+    double s0, sj, si, sij;
+};
+
+struct RunM {
+    int key;
+    double e5, e6, e7, e8, rp, rt, z, w;
+};
+
+}  // namespace
+
+__global__ __launch_bounds__(DMAT_PASS_BLOCK) void dmat_pass_kernel(DmatConst C, DmatForests F,
+                                                                    DmatPairs P, DmatWork W,
+                                                                    DmatOut O) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ unsigned long long s_cnt[6];   // in, sel, count_pass, overflow, negative
+    __shared__ int s_band[4];
+
+    const int s = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    constexpr int NWAVES = DMAT_PASS_BLOCK / 64;
+    const DmatPair q = load_pair(C, F, P, s);
+
+    uint32_t *lU = (uint32_t *)smem;
+    int32_t *l_it = (int32_t *)(lU + C.words_u);
+    int32_t *l_iu = l_it + 2 * C.nt;
+    int32_t *l_jt = l_iu + 2 * C.ntm;
+    int32_t *l_ju = l_jt + 2 * C.nt;
+    const size_t fixed = ((size_t)C.words_u * 4 + (size_t)(4 * C.nt + 4 * C.ntm) * 4 + 15) & ~15;
+    double *lS = (double *)(smem + fixed);            // [4][tb_max][np]
+    double *lE = lS + 4 * C.tb_max * C.np;            // [8][ub_max][npm]
+
+    for (int k = tid; k < C.words_u; k += DMAT_PASS_BLOCK) lU[k] = 0;
+    for (int k = tid; k < 2 * C.nt + 2 * C.ntm; k += DMAT_PASS_BLOCK) {
+        l_it[2 * k] = INT_MAX;
+        l_it[2 * k + 1] = -1;
+    }
+    if (tid < 6) s_cnt[tid] = 0;
+    __syncthreads();
+
+    const double *__restrict__ Ar = F.r + q.oa;
+    const double *__restrict__ Ad = F.d + q.oa;
+    const double *__restrict__ Az = F.z + q.oa;
+    const double *__restrict__ Aw = F.w + q.oa;
+    const double *__restrict__ Al = F.dll + q.oa;
+    const double *__restrict__ Af = F.zfa + q.oa;
+    const double *__restrict__ Br = F.r + q.ob;
+    const double *__restrict__ Bd = F.d + q.ob;
+    const double *__restrict__ Bz = F.z + q.ob;
+    const double *__restrict__ Bw = F.w + q.ob;
+    const double *__restrict__ Bl = F.dll + q.ob;
+    const double *__restrict__ Bf = F.zfb + q.ob;
+    const bool evol = (C.flags & DMAT_REDSHIFT_EVOL) != 0;
+    const int64_t pix = W.pix_off[s];
+
+    // ---- walk 0: census (counts, pixel ranges of every row, U unless truncated) ---------
+    {
+        uint64_t n_in = 0, n_sel = 0, n_cp = 0, n_over = 0, n_neg = 0;
+        for (int c0 = wave * 64; c0 < q.la; c0 += NWAVES * 64) {
+            const int i = c0 + lane;
+            bool active = i < q.la;
+            double r1 = 0, d1 = 0, z1 = 0, w1 = 0;
+            if (active) {
+                r1 = Ar[i];
+                d1 = Ad[i];
+                z1 = Az[i];
+                w1 = Aw[i];
+                active = w1 != 0;
+            }
+            const bool i_sel = !(q.zerr_on && zerr_cut(C, z1, q.zq2));
+            int j_lo, j_hi;
+            window_b(C, q, F, wave_min(active ? r1 : INFINITY),
+                     wave_max(active ? r1 : -INFINITY), wave_min(active ? d1 : INFINITY), lane,
+                     j_lo, j_hi);
+            RowRun rt{-1, 0, 0}, ru{-1, 0, 0};
+            int last_m = -1;
+            uint32_t cnt_in_i = 0;
+            for (int j = j_lo; j < j_hi; ++j) {
+                const double w2 = Bw[j];
+                if (w2 == 0) continue;
+                const double r2 = Br[j], d2 = Bd[j], z2 = Bz[j];
+                const bool j_zerr = q.zerr_on && zerr_cut(C, z2, q.zq1);
+                double r_par, r_trans;
+                int A = 0, M = 0;
+                if (!active || !pair_bins(C, r1, d1, r2, d2, q.cos_half, q.sin_half, r_par,
+                                          r_trans, A, M))
+                    continue;
+                ++n_in;
+                ++cnt_in_i;
+                const bool close = q.shp_cut && fabs(r_par) < C.width_p;
+                if (!close) ++n_cp;
+                if (M < 0 || M >= C.nm) continue;
+                const double z = (z1 + z2) / 2;
+                if (!q.shp_cut && M != last_m) {
+                    atomicOr(&lU[M >> 5], 1u << (M & 31));
+                    last_m = M;
+                }
+                row_step(ru, M % C.ntm, j, l_iu, l_ju, i);
+                if (!pair_selected(C, q, i_sel, j_zerr, z, r_par)) continue;
+                ++n_sel;
+                if (A < 0) {
+                    ++n_neg;
+                    continue;
+                }
+                if (A >= C.nb) {
+                    ++n_over;
+                    continue;
+                }
+                row_step(rt, A % C.nt, j, l_it, l_jt, i);
+            }
+            if (rt.row >= 0) note_run(l_it, l_jt, rt.row, i, rt.first, rt.last);
+            if (ru.row >= 0) note_run(l_iu, l_ju, ru.row, i, ru.first, ru.last);
+            if (q.shp_cut && i < q.la) W.cnt_i[pix + i] = (int32_t)cnt_in_i;
+        }
+        n_in = wave_sum_u64(n_in);
+        n_sel = wave_sum_u64(n_sel);
+        n_cp = wave_sum_u64(n_cp);
+        n_over = wave_sum_u64(n_over);
+        n_neg = wave_sum_u64(n_neg);
+        if (lane == 0) {
+            atomicAdd(&s_cnt[0], (unsigned long long)n_in);
+            atomicAdd(&s_cnt[1], (unsigned long long)n_sel);
+            atomicAdd(&s_cnt[2], (unsigned long long)n_cp);
+            atomicAdd(&s_cnt[3], (unsigned long long)n_over);
+            atomicAdd(&s_cnt[4], (unsigned long long)n_neg);
+        }
+    }
+    __syncthreads();
+    const unsigned long long count_pass = s_cnt[2];
+    // The reference returns before its main pass when the count pass finds no pair
+    // (py/picca/cf.py:572-573): such a forest pair adds nothing.
+    const bool live = count_pass > 0;
+    if (tid == 0) {
+        if (live) {
+            atomicAdd(&W.stats[DMAT_STAT_IN_RANGE], s_cnt[0]);
+            atomicAdd(&W.stats[DMAT_STAT_SELECTED], s_cnt[1]);
+            if (s_cnt[0] > count_pass)
+                atomicAdd(&W.stats[DMAT_STAT_BEYOND], s_cnt[0] - count_pass);
+            atomicAdd(&W.stats[DMAT_STAT_OVERFLOW], s_cnt[3]);
+            atomicAdd(&W.stats[DMAT_STAT_NEGATIVE], s_cnt[4]);
+        }
+        int t_lo = INT_MAX, t_hi = -1, u_lo = INT_MAX, u_hi = -1;
+        if (live) {
+            for (int t = 0; t < C.nt; ++t)
+                if (l_it[2 * t] <= l_it[2 * t + 1]) {
+                    t_lo = min(t_lo, t);
+                    t_hi = max(t_hi, t);
+                }
+            for (int u = 0; u < C.ntm; ++u)
+                if (l_iu[2 * u] <= l_iu[2 * u + 1]) {
+                    u_lo = min(u_lo, u);
+                    u_hi = max(u_hi, u);
+                }
+        }
+        if (t_hi < 0 || u_hi < 0) {   // nothing selected or no model bin: no fill
+            t_lo = u_lo = 1;
+            t_hi = u_hi = 0;
+        }
+        s_band[0] = t_lo;
+        s_band[1] = t_hi;
+        s_band[2] = u_lo;
+        s_band[3] = u_hi;
+        W.band[4 * s + 0] = t_lo;
+        W.band[4 * s + 1] = t_hi;
+        W.band[4 * s + 2] = u_lo;
+        W.band[4 * s + 3] = u_hi;
+    }
+    // pixel ranges of every row
+    for (int k = tid; k < 2 * C.nt; k += DMAT_PASS_BLOCK) {
+        W.rng_it[(int64_t)s * 2 * C.nt + k] = l_it[k];
+        W.rng_jt[(int64_t)s * 2 * C.nt + k] = l_jt[k];
+    }
+    for (int k = tid; k < 2 * C.ntm; k += DMAT_PASS_BLOCK) {
+        W.rng_iu[(int64_t)s * 2 * C.ntm + k] = l_iu[k];
+        W.rng_ju[(int64_t)s * 2 * C.ntm + k] = l_ju[k];
+    }
+    // (t, u) blocks this pair feeds
+    if (live) {
+        for (int k = tid; k < C.nt * C.ntm; k += DMAT_PASS_BLOCK) {
+            const int t = k / C.ntm, u = k - t * C.ntm;
+            if (l_it[2 * t] <= l_it[2 * t + 1] && l_iu[2 * u] <= l_iu[2 * u + 1])
+                atomicOr(&W.touched[k >> 5], 1u << (k & 31));
+        }
+    }
+    if (!live) return;   // uniform over the workgroup
+
+    // ---- unique model bins of the first count_pass in-range pairs ----------------------
+    // With same-half-plate removal the reference sizes its model-bin record by the count
+    // pass, which skips close pairs, but fills it with every in-range pair in (i, j) order
+    // and takes the unique values of the first count_pass entries (py/picca/cf.py:619,
+    // 702-703, 874): U is the model bins of the first count_pass in-range pairs.
+    if (q.shp_cut) {
+        if (wave == 0) {   // exclusive prefix of the per-pixel in-range counts
+            int64_t base = 0;
+            for (int c0 = 0; c0 < q.la; c0 += 64) {
+                const int i = c0 + lane;
+                const int64_t v = i < q.la ? W.cnt_i[pix + i] : 0;
+                int64_t incl = v;
+                for (int off = 1; off < 64; off <<= 1) {
+                    const int64_t o = __shfl_up(incl, off, 64);
+                    if (lane >= off) incl += o;
+                }
+                if (i < q.la) W.cnt_i[pix + i] = (int32_t)(base + incl - v);
+                base += __shfl(incl, 63, 64);
+            }
+        }
+        __syncthreads();
+        for (int c0 = wave * 64; c0 < q.la; c0 += NWAVES * 64) {
+            const int i = c0 + lane;
+            bool active = i < q.la;
+            double r1 = 0, d1 = 0, w1 = 0;
+            int64_t g = 0;
+            if (active) {
+                r1 = Ar[i];
+                d1 = Ad[i];
+                w1 = Aw[i];
+                g = W.cnt_i[pix + i];
+                active = w1 != 0 && g < (int64_t)count_pass;
+            }
+            int j_lo, j_hi;
+            window_b(C, q, F, wave_min(active ? r1 : INFINITY),
+                     wave_max(active ? r1 : -INFINITY), wave_min(active ? d1 : INFINITY), lane,
+                     j_lo, j_hi);
+            for (int j = j_lo; j < j_hi; ++j) {
+                const double w2 = Bw[j];
+                if (w2 == 0) continue;
+                double r_par, r_trans;
+                int A = 0, M = 0;
+                if (!active || !pair_bins(C, r1, d1, Br[j], Bd[j], q.cos_half, q.sin_half,
+                                          r_par, r_trans, A, M))
+                    continue;
+                if (g < (int64_t)count_pass && M >= 0 && M < C.nm)
+                    atomicOr(&lU[M >> 5], 1u << (M & 31));
+                ++g;
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- accumulation passes over LDS bands of r_trans rows ------------------------------
+    const int t_lo = s_band[0], t_hi = s_band[1], u_lo = s_band[2], u_hi = s_band[3];
+    const int n_t = t_hi >= t_lo ? t_hi - t_lo + 1 : 0;
+    const int n_u = u_hi >= u_lo ? u_hi - u_lo + 1 : 0;
+    const int passes = max((n_t + C.tb_max - 1) / C.tb_max, (n_u + C.ub_max - 1) / C.ub_max);
+    for (int pass = 0; pass < passes; ++pass) {
+        const int tc0 = t_lo + pass * C.tb_max;
+        const int uc0 = u_lo + pass * C.ub_max;
+        for (int k = tid; k < 4 * C.tb_max * C.np; k += DMAT_PASS_BLOCK) lS[k] = 0;
+        for (int k = tid; k < 8 * C.ub_max * C.npm; k += DMAT_PASS_BLOCK) lE[k] = 0;
+        __syncthreads();
+        auto flush_a = [&](const RunA &ra) {
+            const int t = ra.key % C.nt;
+            if (t < tc0 || t >= tc0 + C.tb_max) return;
+            double *b = lS + (t - tc0) * C.np + ra.key / C.nt;
+            const int stride = C.tb_max * C.np;
+            lds_add(b, ra.s0);
+            lds_add(b + stride, ra.sj);
+            lds_add(b + 2 * stride, ra.si);
+            lds_add(b + 3 * stride, ra.sij);
+        };
+        auto flush_m = [&](const RunM &rm) {
+            const int u = rm.key % C.ntm;
+            if (u < uc0 || u >= uc0 + C.ub_max) return;
+            double *b = lE + (u - uc0) * C.npm + rm.key / C.ntm;
+            const int stride = C.ub_max * C.npm;
+            lds_add(b, rm.e5);
+            lds_add(b + stride, rm.e6);
+            lds_add(b + 2 * stride, rm.e7);
+            lds_add(b + 3 * stride, rm.e8);
+            lds_add(b + 4 * stride, rm.rp);
+            lds_add(b + 5 * stride, rm.rt);
+            lds_add(b + 6 * stride, rm.z);
+            lds_add(b + 7 * stride, rm.w);
+        };
+        for (int c0 = wave * 64; c0 < q.la; c0 += NWAVES * 64) {
+            const int i = c0 + lane;
+            bool active = i < q.la;
+            double r1 = 0, d1 = 0, z1 = 0, w1 = 0, l1 = 0, f1 = 1;
+            if (active) {
+                r1 = Ar[i];
+                d1 = Ad[i];
+                z1 = Az[i];
+                w1 = Aw[i];
+                l1 = Al[i];
+                if (evol) f1 = Af[i];
+                active = w1 != 0;
+            }
+            const bool i_sel = !(q.zerr_on && zerr_cut(C, z1, q.zq2));
+            // per-pixel factors of the eta sums (py/picca/cf.py:765-843)
+            const double a7 = q.order1 ? w1 * l1 / q.sw1 : 0.0;    // w1 dll1 / sum w dll^2
+            int j_lo, j_hi;
+            window_b(C, q, F, wave_min(active ? r1 : INFINITY),
+                     wave_max(active ? r1 : -INFINITY), wave_min(active ? d1 : INFINITY), lane,
+                     j_lo, j_hi);
+            RunA ra{-1, 0, 0, 0, 0};
+            RunM rm{-1, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int j = j_lo; j < j_hi; ++j) {
+                const double w2 = Bw[j];
+                if (w2 == 0) continue;
+                const double r2 = Br[j], d2 = Bd[j], z2 = Bz[j], l2 = Bl[j];
+                const double f2 = evol ? Bf[j] : 1.0;
+                const bool j_zerr = q.zerr_on && zerr_cut(C, z2, q.zq1);
+                double r_par, r_trans;
+                int A = 0, M = 0;
+                if (!active || !pair_bins(C, r1, d1, r2, d2, q.cos_half, q.sin_half, r_par,
+                                          r_trans, A, M))
+                    continue;
+                if (M < 0 || M >= C.nm) continue;
+                const double z = (z1 + z2) / 2;
+                const double w12 = w1 * w2;
+                const double zfac = evol ? f1 * f2 : 1.0;
+                const bool sel = pair_selected(C, q, i_sel, j_zerr, z, r_par);
+                if (M != rm.key) {
+                    if (rm.key >= 0) flush_m(rm);
+                    rm = RunM{M, 0, 0, 0, 0, 0, 0, 0, 0};
+                }
+                rm.e5 += zfac * w12 / q.sum_w1 / q.sum_w2;
+                if (q.order2) rm.e6 += zfac * w1 / q.sum_w1 * (w2 * l2 / q.sw2);
+                if (q.order1) {
+                    rm.e7 += zfac * w2 / q.sum_w2 * a7;
+                    if (q.order2) rm.e8 += zfac * w1 * l1 * w2 * l2 / q.sw1 / q.sw2;
+                }
+                if (!sel) continue;
+                rm.rp += w12 * r_par;
+                rm.rt += w12 * r_trans;
+                rm.z += w12 * z;
+                rm.w += w12;
+                if (A < 0 || A >= C.nb) continue;
+                if (A != ra.key) {
+                    if (ra.key >= 0) flush_a(ra);
+                    ra = RunA{A, 0, 0, 0, 0};
+                }
+                ra.s0 += w12;
+                ra.sj += w12 * l2;
+                ra.si += w12 * l1;
+                ra.sij += w12 * l1 * l2;
+            }
+            if (ra.key >= 0) flush_a(ra);
+            if (rm.key >= 0) flush_m(rm);
+        }
+        __syncthreads();
+        // band rows -> workspace (eta masked by U), weights_dmat and effective sums -> out
+        const int t_end = min(t_hi + 1, tc0 + C.tb_max);
+        for (int k = tid; k < (t_end - tc0) * C.np; k += DMAT_PASS_BLOCK) {
+            const int tt = k / C.np, rp = k - tt * C.np, t = tc0 + tt;
+            const double *b = lS + tt * C.np + rp;
+            const int stride = C.tb_max * C.np;
+            for (int c = 0; c < 4; ++c)
+                W.S[(((int64_t)s * 4 + c) * C.nt + t) * C.np + rp] = b[c * stride];
+            if (b[0] != 0) atomicAdd(&O.weights_dmat[rp * C.nt + t], b[0]);
+        }
+        const int u_end = min(u_hi + 1, uc0 + C.ub_max);
+        for (int k = tid; k < (u_end - uc0) * C.npm; k += DMAT_PASS_BLOCK) {
+            const int uu = k / C.npm, mrp = k - uu * C.npm, u = uc0 + uu;
+            const double *b = lE + uu * C.npm + mrp;
+            const int stride = C.ub_max * C.npm;
+            const int kb = mrp * C.ntm + u;
+            const double mask = ((lU[kb >> 5] >> (kb & 31)) & 1u) ? 1.0 : 0.0;
+            for (int c = 0; c < 4; ++c)
+                W.eta[(((int64_t)s * 4 + c) * C.ntm + u) * C.npm + mrp] = b[c * stride] * mask;
+            if (b[4 * stride] != 0 || b[5 * stride] != 0 || b[6 * stride] != 0 ||
+                b[7 * stride] != 0) {
+                atomicAdd(&O.r_par_eff[kb], b[4 * stride]);
+                atomicAdd(&O.r_trans_eff[kb], b[5 * stride]);
+                atomicAdd(&O.z_eff[kb], b[6 * stride]);
+                atomicAdd(&O.weight_eff[kb], b[7 * stride]);
+            }
+        }
+        __syncthreads();
+    }
+    for (int k = tid; k < C.words_u; k += DMAT_PASS_BLOCK)
+        W.U[(int64_t)s * C.words_u + k] = lU[k];
+}
+
+namespace {
+
+// per-wave vectors of the fill kernel
+constexpr int VEC_P = 0;                                  // P (or Q)       [CHUNK_RP]
+constexpr int VEC_P2 = DMAT_CHUNK_RP;                     // P' (or Q')     [CHUNK_RP]
+constexpr int VEC_E = 2 * DMAT_CHUNK_RP;                  // eta1 (or eta2) [CHUNK_MRP]
+constexpr int VEC_E2 = 2 * DMAT_CHUNK_RP + DMAT_CHUNK_MRP;  // eta3 (or eta4)
+constexpr int VEC_LEN = 2 * DMAT_CHUNK_RP + 2 * DMAT_CHUNK_MRP;
+constexpr int FILL_WAVES = DMAT_FILL_BLOCK / 64;
+
+constexpr size_t FILL_LDS_DOUBLES = (size_t)DMAT_CHUNK_RP * DMAT_CHUNK_MRP   // diag block
+                                    + 4 * DMAT_CHUNK_RP + 4 * DMAT_CHUNK_MRP  // S, eta rows
+                                    + DMAT_CHUNK_MRP                          // U mask
+                                    + (size_t)FILL_WAVES * VEC_LEN;          // vectors
+
+}  // namespace
+
+__global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C, DmatForests F,
+                                                                    DmatPairs P, DmatWork W,
+                                                                    DmatOut O, int groups) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tile = blockIdx.x;
+    if (!((W.touched[tile >> 5] >> (tile & 31)) & 1u)) return;
+    const int t = tile / C.ntm;
+    const int u = tile - t * C.ntm;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int tx = tid & 15;   // owner of model r_par bins tx + 16 b
+    const int ty = tid >> 4;   // owner of data r_par bins ty + 16 a
+    const bool evol = (C.flags & DMAT_REDSHIFT_EVOL) != 0;
+
+    double *diag = (double *)smem;                              // [CHUNK_RP][CHUNK_MRP]
+    double *vS = diag + DMAT_CHUNK_RP * DMAT_CHUNK_MRP;         // [4][CHUNK_RP]
+    double *vE = vS + 4 * DMAT_CHUNK_RP;                        // [4][CHUNK_MRP]
+    double *mask = vE + 4 * DMAT_CHUNK_MRP;                     // [CHUNK_MRP]
+    double *vecs = mask + DMAT_CHUNK_MRP;                       // [FILL_WAVES][VEC_LEN]
+    double *vec = vecs + wave * VEC_LEN;
+
+    for (int rp0 = 0; rp0 < C.np; rp0 += DMAT_CHUNK_RP) {
+        for (int mrp0 = 0; mrp0 < C.npm; mrp0 += DMAT_CHUNK_MRP) {
+            const int nrp = min(DMAT_CHUNK_RP, C.np - rp0);
+            const int nmrp = min(DMAT_CHUNK_MRP, C.npm - mrp0);
+            double acc[DMAT_FILL_RP][DMAT_FILL_MRP];
+#pragma unroll
+            for (int a = 0; a < DMAT_FILL_RP; ++a)
+#pragma unroll
+                for (int b = 0; b < DMAT_FILL_MRP; ++b) acc[a][b] = 0;
+            for (int k = tid; k < DMAT_CHUNK_RP * DMAT_CHUNK_MRP; k += DMAT_FILL_BLOCK)
+                diag[k] = 0;
+            for (int k = tid; k < FILL_WAVES * VEC_LEN; k += DMAT_FILL_BLOCK) vecs[k] = 0;
+            __syncthreads();
+
+            auto outer = [&](int nvec) {
+                // acc -= sum over the first nvec waves' vectors of P x E + P2 x E2
+                for (int w = 0; w < nvec; ++w) {
+                    const double *v = vecs + w * VEC_LEN;
+                    double p[DMAT_FILL_RP], p2[DMAT_FILL_RP];
+#pragma unroll
+                    for (int a = 0; a < DMAT_FILL_RP; ++a) {
+                        p[a] = v[VEC_P + ty + 16 * a];
+                        p2[a] = v[VEC_P2 + ty + 16 * a];
+                    }
+#pragma unroll
+                    for (int b = 0; b < DMAT_FILL_MRP; ++b) {
+                        const double e = v[VEC_E + tx + 16 * b];
+                        const double e2 = v[VEC_E2 + tx + 16 * b];
+#pragma unroll
+                        for (int a = 0; a < DMAT_FILL_RP; ++a)
+                            acc[a][b] -= p[a] * e + p2[a] * e2;
+                    }
+                }
+            };
+
+            for (int s = blockIdx.y; s < P.count; s += groups) {
+                const int t_lo = W.band[4 * s], t_hi = W.band[4 * s + 1];
+                const int u_lo = W.band[4 * s + 2], u_hi = W.band[4 * s + 3];
+                if (t < t_lo || t > t_hi || u < u_lo || u > u_hi) continue;
+                const DmatPair q = load_pair(C, F, P, s);
+
+                // rank-4 term S[., t] x eta5..8[., u] (eta already masked by U)
+                for (int k = tid; k < 4 * DMAT_CHUNK_RP; k += DMAT_FILL_BLOCK) {
+                    const int c = k / DMAT_CHUNK_RP, r = k - c * DMAT_CHUNK_RP;
+                    vS[k] = r < nrp ? W.S[(((int64_t)s * 4 + c) * C.nt + t) * C.np + rp0 + r]
+                                    : 0.0;
+                }
+                for (int k = tid; k < 4 * DMAT_CHUNK_MRP; k += DMAT_FILL_BLOCK) {
+                    const int c = k / DMAT_CHUNK_MRP, m = k - c * DMAT_CHUNK_MRP;
+                    vE[k] = m < nmrp
+                                ? W.eta[(((int64_t)s * 4 + c) * C.ntm + u) * C.npm + mrp0 + m]
+                                : 0.0;
+                }
+                for (int m = tid; m < DMAT_CHUNK_MRP; m += DMAT_FILL_BLOCK) {
+                    double v = 0.0;
+                    if (m < nmrp) {
+                        const int kb = (mrp0 + m) * C.ntm + u;
+                        v = ((W.U[(int64_t)s * C.words_u + (kb >> 5)] >> (kb & 31)) & 1u) ? 1.0
+                                                                                          : 0.0;
+                    }
+                    mask[m] = v;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    double sv[DMAT_FILL_RP];
+#pragma unroll
+                    for (int a = 0; a < DMAT_FILL_RP; ++a) sv[a] = vS[c * DMAT_CHUNK_RP + ty + 16 * a];
+#pragma unroll
+                    for (int b = 0; b < DMAT_FILL_MRP; ++b) {
+                        const double e = vE[c * DMAT_CHUNK_MRP + tx + 16 * b];
+#pragma unroll
+                        for (int a = 0; a < DMAT_FILL_RP; ++a) acc[a][b] += sv[a] * e;
+                    }
+                }
+
+                const double *__restrict__ Ar = F.r + q.oa;
+                const double *__restrict__ Ad = F.d + q.oa;
+                const double *__restrict__ Az = F.z + q.oa;
+                const double *__restrict__ Aw = F.w + q.oa;
+                const double *__restrict__ Al = F.dll + q.oa;
+                const double *__restrict__ Af = F.zfa + q.oa;
+                const double *__restrict__ Br = F.r + q.ob;
+                const double *__restrict__ Bd = F.d + q.ob;
+                const double *__restrict__ Bz = F.z + q.ob;
+                const double *__restrict__ Bw = F.w + q.ob;
+                const double *__restrict__ Bl = F.dll + q.ob;
+                const double *__restrict__ Bf = F.zfb + q.ob;
+
+                // ---- pixels i of delta1: P, P' (row t) and eta1, eta3 (row u) ----------
+                {
+                    const int32_t *it = W.rng_it + ((int64_t)s * C.nt + t) * 2;
+                    const int32_t *iu = W.rng_iu + ((int64_t)s * C.ntm + u) * 2;
+                    const int i0 = max(it[0], iu[0]);
+                    const int i1 = min(it[1], iu[1]);
+                    for (int base = i0; base <= i1; base += FILL_WAVES) {
+                        const int i = base + wave;
+                        if (i <= i1) {
+                            const double w1 = Aw[i];
+                            if (w1 != 0) {
+                                const double r1 = Ar[i], d1 = Ad[i], z1 = Az[i], l1 = Al[i];
+                                const double f1 = evol ? Af[i] : 1.0;
+                                const bool i_sel = !(q.zerr_on && zerr_cut(C, z1, q.zq2));
+                                int j_lo, j_hi;
+                                window_b(C, q, F, r1, r1, d1, lane, j_lo, j_hi);
+                                for (int j = j_lo + lane; j < j_hi; j += 64) {
+                                    const double w2 = Bw[j];
+                                    if (w2 == 0) continue;
+                                    double r_par, r_trans;
+                                    int A = 0, M = 0;
+                                    if (!pair_bins(C, r1, d1, Br[j], Bd[j], q.cos_half,
+                                                   q.sin_half, r_par, r_trans, A, M))
+                                        continue;
+                                    if (M < 0 || M >= C.nm) continue;
+                                    const double z2 = Bz[j], l2 = Bl[j];
+                                    const double zfac = evol ? f1 * Bf[j] : 1.0;
+                                    const int mu = M % C.ntm;
+                                    const int m = M / C.ntm - mrp0;
+                                    const bool in_m = mu == u && m >= 0 && m < nmrp;
+                                    if (in_m && mask[m] != 0) {
+                                        lds_add(vec + VEC_E + m, zfac * w2 / q.sum_w2);
+                                        if (q.order2)
+                                            lds_add(vec + VEC_E2 + m, zfac * w2 * l2 / q.sw2);
+                                    }
+                                    const bool j_zerr = q.zerr_on && zerr_cut(C, z2, q.zq1);
+                                    if (!pair_selected(C, q, i_sel, j_zerr, (z1 + z2) / 2, r_par))
+                                        continue;
+                                    if (A < 0 || A >= C.nb || A % C.nt != t) continue;
+                                    const int r = A / C.nt - rp0;
+                                    if (r < 0 || r >= nrp) continue;
+                                    const double w12 = w1 * w2;
+                                    lds_add(vec + VEC_P + r, w12);
+                                    lds_add(vec + VEC_P2 + r, w12 * l2);
+                                    if (in_m) lds_add(diag + r * DMAT_CHUNK_MRP + m, w12 * zfac);
+                                }
+                            }
+                        }
+                        __syncthreads();
+                        outer(min(FILL_WAVES, i1 - base + 1));
+                        __syncthreads();
+                        for (int k = lane; k < VEC_LEN; k += 64) vec[k] = 0;
+                    }
+                }
+                // ---- pixels j of delta2: Q, Q' (row t) and eta2, eta4 (row u) ----------
+                {
+                    const int32_t *jt = W.rng_jt + ((int64_t)s * C.nt + t) * 2;
+                    const int32_t *ju = W.rng_ju + ((int64_t)s * C.ntm + u) * 2;
+                    const int j0 = max(jt[0], ju[0]);
+                    const int j1 = min(jt[1], ju[1]);
+                    for (int base = j0; base <= j1; base += FILL_WAVES) {
+                        const int j = base + wave;
+                        if (j <= j1) {
+                            const double w2 = Bw[j];
+                            if (w2 != 0) {
+                                const double r2 = Br[j], d2 = Bd[j], z2 = Bz[j];
+                                const double f2 = evol ? Bf[j] : 1.0;
+                                const bool j_zerr = q.zerr_on && zerr_cut(C, z2, q.zq1);
+                                int i_lo, i_hi;
+                                window_a(C, q, F, r2, d2, lane, i_lo, i_hi);
+                                for (int i = i_lo + lane; i < i_hi; i += 64) {
+                                    const double w1 = Aw[i];
+                                    if (w1 == 0) continue;
+                                    double r_par, r_trans;
+                                    int A = 0, M = 0;
+                                    if (!pair_bins(C, Ar[i], Ad[i], r2, d2, q.cos_half,
+                                                   q.sin_half, r_par, r_trans, A, M))
+                                        continue;
+                                    if (M < 0 || M >= C.nm) continue;
+                                    const double z1 = Az[i], l1 = Al[i];
+                                    const double zfac = evol ? Af[i] * f2 : 1.0;
+                                    const int mu = M % C.ntm;
+                                    const int m = M / C.ntm - mrp0;
+                                    if (mu == u && m >= 0 && m < nmrp && mask[m] != 0) {
+                                        lds_add(vec + VEC_E + m, zfac * w1 / q.sum_w1);
+                                        if (q.order1)
+                                            lds_add(vec + VEC_E2 + m, zfac * w1 * l1 / q.sw1);
+                                    }
+                                    const bool i_sel = !(q.zerr_on && zerr_cut(C, z1, q.zq2));
+                                    if (!pair_selected(C, q, i_sel, j_zerr, (z1 + z2) / 2, r_par))
+                                        continue;
+                                    if (A < 0 || A >= C.nb || A % C.nt != t) continue;
+                                    const int r = A / C.nt - rp0;
+                                    if (r < 0 || r >= nrp) continue;
+                                    const double w12 = w1 * w2;
+                                    lds_add(vec + VEC_P + r, w12);
+                                    lds_add(vec + VEC_P2 + r, w12 * l1);
+                                }
+                            }
+                        }
+                        __syncthreads();
+                        outer(min(FILL_WAVES, j1 - base + 1));
+                        __syncthreads();
+                        for (int k = lane; k < VEC_LEN; k += 64) vec[k] = 0;
+                    }
+                }
+                __syncthreads();   // vS / vE / mask are rewritten for the next pair
+            }
+
+            // block (+ the pair-bin term) -> dmat
+#pragma unroll
+            for (int a = 0; a < DMAT_FILL_RP; ++a) {
+                const int r = ty + 16 * a;
+#pragma unroll
+                for (int b = 0; b < DMAT_FILL_MRP; ++b) {
+                    const int m = tx + 16 * b;
+                    if (r < nrp && m < nmrp) {
+                        const double v = acc[a][b] + diag[r * DMAT_CHUNK_MRP + m];
+                        if (v != 0) {
+                            const int64_t A = (int64_t)(rp0 + r) * C.nt + t;
+                            const int64_t k = (int64_t)(mrp0 + m) * C.ntm + u;
+                            atomicAdd(&O.dmat[A * C.nm + k], v);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+size_t dmat_fill_lds(const DmatConst &) { return FILL_LDS_DOUBLES * sizeof(double); }
+
+hipError_t launch_dmat_pass(const DmatConst &C, const DmatForests &F, const DmatPairs &P,
+                            const DmatWork &W, const DmatOut &O, size_t lds,
+                            hipStream_t stream) {
+    if (P.count <= 0) return hipSuccess;
+    hipError_t err = hipFuncSetAttribute((const void *)dmat_pass_kernel,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(dmat_pass_kernel, dim3(P.count), dim3(DMAT_PASS_BLOCK), lds, stream, C,
+                       F, P, W, O);
+    return hipGetLastError();
+}
+
+hipError_t launch_dmat_fill(const DmatConst &C, const DmatForests &F, const DmatPairs &P,
+                            const DmatWork &W, const DmatOut &O, int groups,
+                            hipStream_t stream) {
+    if (P.count <= 0) return hipSuccess;
+    hipError_t err = hipFuncSetAttribute((const void *)dmat_fill_kernel,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)dmat_fill_lds(C));
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(dmat_fill_kernel, dim3(C.nt * C.ntm, groups), dim3(DMAT_FILL_BLOCK),
+                       dmat_fill_lds(C), stream, C, F, P, W, O, groups);
+    return hipGetLastError();
+}
+
+}  // namespace picca

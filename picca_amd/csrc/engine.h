@@ -107,9 +107,107 @@ struct XiOut {
 constexpr size_t XI_HIST_BIN_BYTES = 48;
 constexpr size_t XI_TILE_PIXEL_BYTES = 64;
 
+// ---------------------------------------------------------------------------------------
+// distortion matrix
+// ---------------------------------------------------------------------------------------
+
+constexpr int DMAT_PASS_BLOCK = 256;   // threads of the pair-pass kernel (lane per pixel i)
+constexpr int DMAT_FILL_BLOCK = 256;   // threads of the fill kernel (16 x 16 block owners)
+constexpr int DMAT_FILL_RP = 4;        // data r_par bins per owner thread  (64 per chunk)
+constexpr int DMAT_FILL_MRP = 7;       // model r_par bins per owner thread (112 per chunk)
+constexpr int DMAT_CHUNK_RP = 16 * DMAT_FILL_RP;
+constexpr int DMAT_CHUNK_MRP = 16 * DMAT_FILL_MRP;
+
+enum : int {
+    DMAT_REDSHIFT_EVOL = 32,
+    DMAT_XCORR = 64,
+    DMAT_RMU = 128,
+};
+
+enum : int {
+    DMAT_STAT_IN_RANGE = 0,     // in-range pairs of the main pass
+    DMAT_STAT_SELECTED = 1,     // selected pairs with a valid model bin
+    DMAT_STAT_BEYOND = 2,       // in-range pairs beyond the count-pass size
+    DMAT_STAT_OVERFLOW = 3,     // selected pairs whose data bin is >= np*nt
+    DMAT_STAT_NEGATIVE = 4,     // selected pairs whose data bin is < 0
+    DMAT_NSTATS = 5,
+};
+
+struct DmatConst {
+    int32_t np, nt, nb;        // data bins
+    int32_t npm, ntm, nm;      // model bins
+    int32_t flags;             // XI_HAS_* | XI_REMOVE_SHP | DMAT_*
+    int32_t words_u;           // 32-bit words of a model-bin bitmap
+    int32_t tb_max, ub_max;    // r_trans bins of the LDS bands of the pass kernel
+    double rpmin, rpmax, rtmax, span_p, width_p;
+    double zmin, zmax, zerr_ang, zerr_kms;
+    double np_d, nt_d, npm_d, ntm_d;
+};
+
+// Per-pixel and per-forest inputs of the distortion matrix (store order).
+struct DmatForests {
+    const double *__restrict__ r;
+    const double *__restrict__ d;
+    const double *__restrict__ z;
+    const double *__restrict__ w;
+    const double *__restrict__ dll;   // log_lambda - weighted mean log_lambda of the forest
+    const double *__restrict__ zfa;   // ((1+z)/(1+z_ref))^(alpha-1)
+    const double *__restrict__ zfb;   // ((1+z)/(1+z_ref))^(alpha2-1)
+    const int64_t *__restrict__ off;
+    const double *__restrict__ zq;
+    const double *__restrict__ sum_w;   // per forest: sum of weights
+    const double *__restrict__ sw_dll;  // per forest: sum of w * dll^2
+    const int32_t *__restrict__ order;
+    const uint8_t *__restrict__ mono;
+};
+
+// One batch of kept forest pairs.
+struct DmatPairs {
+    const int32_t *__restrict__ a;
+    const int32_t *__restrict__ b;
+    const double *__restrict__ ang;
+    const double *__restrict__ cs;     // cos(ang/2), sin(ang/2)
+    const uint8_t *__restrict__ flags;
+    int32_t count;
+};
+
+// Per-slot workspace of a batch (slot = position of the pair in the batch).
+// Data r_trans bin t / model r_trans bin u below are the remainders A % nt and M % ntm of
+// the reference's flat bin indices A and M (so a bin index that rounds to nt lands where
+// the reference writes it).
+struct DmatWork {
+    double *S;          // [slot][4][nt][np]   S0, Sj, Si, Sij per data bin (band only)
+    double *eta;        // [slot][4][ntm][npm] eta5..eta8 per model bin, masked by U (band)
+    uint32_t *U;        // [slot][words_u]     unique model bins of the fill loop
+    int32_t *band;      // [slot][4]           t_lo, t_hi, u_lo, u_hi (empty: lo > hi)
+    // pixel ranges per slot, [slot][bins][2] = (first, last) pixel index, first > last
+    // when empty: pixels i of delta1 with a selected pair in data bin row t (it) / an
+    // in-range pair in model bin row u (iu); the same for pixels j of delta2 (jt, ju)
+    int32_t *rng_it, *rng_iu, *rng_jt, *rng_ju;
+    int32_t *cnt_i;     // [pix_off[slot] + i] in-range pairs of pixel i (same-half-plate)
+    const int64_t *pix_off;
+    uint32_t *touched;  // [(nt*ntm + 31) / 32] (t, u) blocks some slot covers
+    unsigned long long *stats;  // [DMAT_NSTATS]
+};
+
+struct DmatOut {
+    double *weights_dmat;   // [nb]
+    double *dmat;           // [nb][nm]
+    double *r_par_eff, *r_trans_eff, *z_eff, *weight_eff;  // [nm]
+};
+
+hipError_t launch_dmat_pass(const DmatConst &C, const DmatForests &F, const DmatPairs &P,
+                            const DmatWork &W, const DmatOut &O, size_t lds,
+                            hipStream_t stream);
+hipError_t launch_dmat_fill(const DmatConst &C, const DmatForests &F, const DmatPairs &P,
+                            const DmatWork &W, const DmatOut &O, int groups,
+                            hipStream_t stream);
+size_t dmat_fill_lds(const DmatConst &C);
+
 hipError_t launch_xi(int mode, bool xcorr, bool general, bool lds_hist, const XiConst &C,
                      const ForestView &F, const PairView &P, const GroupView &G,
                      const XiOut &O, int grid, size_t lds, hipStream_t stream);
 hipError_t launch_zero_u64(unsigned long long *ptr, int64_t n, hipStream_t stream);
+hipError_t launch_copy16(const void *src, void *dst, int64_t n16, hipStream_t stream);
 
 }  // namespace picca

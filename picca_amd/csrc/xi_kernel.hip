@@ -34,61 +34,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "device_util.h"
 #include "engine.h"
 
 namespace picca {
-
-__device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-__device__ __forceinline__ int64_t uniform_i64(int64_t v) {
-    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
-    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
-    return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-
-__device__ __forceinline__ double uniform_f64(double v) {
-    uint64_t bits = __double_as_longlong(v);
-    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)bits);
-    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(bits >> 32));
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
-
-__device__ __forceinline__ double wave_min(double v) {
-    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
-__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// First index in [0, n) with arr[idx] >= x (n if none) when `strict`, or with
-// arr[idx] > x otherwise; arr must be non-decreasing.  All 64 lanes probe one level at a
-// time, so the search takes ceil(log64 n) rounds.
-template <bool STRICT>
-__device__ int wave_search(const double *__restrict__ arr, int n, double x, int lane) {
-    int lo = 0, len = n;
-    while (len > 0) {
-        const int stride = (len + 63) >> 6;
-        const int probe = lane * stride;
-        bool below = false;
-        if (probe < len) below = STRICT ? (arr[lo + probe] < x) : (arr[lo + probe] <= x);
-        const int cnt = __popcll(__ballot(below));
-        if (cnt == 0) break;
-        const int start = lo + (cnt - 1) * stride + 1;
-        int stop = lo + cnt * stride;
-        if (stop > lo + len) stop = lo + len;
-        lo = start;
-        len = stop - start;
-    }
-    return lo;
-}
 
 __device__ __forceinline__ double edge_distance(double t, double width) {
     double frac = t - floor(t);
@@ -633,6 +582,22 @@ __global__ void zero_u64_kernel(unsigned long long *__restrict__ a, int64_t n) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (; k < n; k += stride) a[k] = 0ull;
+}
+
+__global__ __launch_bounds__(256) void copy16_kernel(const uint4 *__restrict__ src,
+                                                    uint4 *__restrict__ dst, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+        dst[k] = src[k];
+}
+
+hipError_t launch_copy16(const void *src, void *dst, int64_t n16, hipStream_t stream) {
+    if (n16 <= 0) return hipSuccess;
+    int64_t blocks = (n16 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    copy16_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>((const uint4 *)src,
+                                                                    (uint4 *)dst, n16);
+    return hipGetLastError();
 }
 
 hipError_t launch_zero_u64(unsigned long long *ptr, int64_t n, hipStream_t stream) {

@@ -158,6 +158,13 @@ class Engine:
         (we, xi, rp, rt, z), nb = self.xi_download(params)
         return XiResult(we, xi, rp, rt, z, nb, report.as_dict())
 
+    def diag_copy(self, nbytes):
+        """Plain device copy of nbytes (multiple of 16); returns the kernel time in ms."""
+        ms = ctypes.c_double()
+        check(self._lib.picca_diag_copy(self._ctx, int(nbytes), ctypes.byref(ms)),
+              "picca_diag_copy")
+        return ms.value
+
     # -- dmat -----------------------------------------------------------------
     def dmat(self, params, pair_a, pair_b, pair_ang, pair_flags):
         nb = params.num_bins_r_par * params.num_bins_r_trans
