@@ -146,6 +146,40 @@ int picca_dmat_run(picca_ctx *ctx, const picca_cf_params *params, int64_t num_pa
                    double *weight_eff, int64_t *stats);
 
 /*
+ * Quasar-continuum fit (compute_continuum,
+ * py/picca/delta_extraction/expected_fluxes/utils.py:12-136, with the objective of
+ * LeastsSquaresContModel, py/picca/delta_extraction/least_squares/
+ * least_squares_cont_model.py:65-197), batched over forests.
+ *
+ * picca_cont_set_forests makes the fit inputs resident: forest k owns pixels
+ * [offsets[k], offsets[k+1]); per pixel the flux, ivar, the mean continuum times the
+ * transmission correction (mean_cont), x = (log_lambda - Lmin) / (Lmax - Lmin) with
+ * Lmin/Lmax the rest-frame grid ends shifted to the forest's redshift (:59-78), and the
+ * variance terms eta, var_lss, fudge evaluated at the pixel (:80-86); per forest the
+ * polynomial order (0 fixes the slope at 0).
+ *
+ * picca_cont_fit minimises chi2(a, b) for every forest from the start (a0, b0) given per
+ * forest (compute_continuum starts at a0 = sum f ivar / sum ivar, b0 = 0) and returns a
+ * (zero point), b (slope), the minimum chi2, ndata and a status: 0 fitted, 1 not
+ * converged within max_iter, 2 negative continuum, 3 zero continuum, 4 non-finite chi2
+ * (the reference's bad_continuum_reason cases, :111-120), and the number of objective
+ * evaluations.
+ */
+typedef struct picca_cont_params {
+    int32_t max_iter;             /* Levenberg-Marquardt iterations (e.g. 100)          */
+    int32_t use_constant_weight;  /* 1: all weights 1 (chi2 = sum of squares)           */
+    double tol;                   /* stop when g^T H^-1 g / 2 < tol * max(1, |chi2|)    */
+} picca_cont_params;
+
+int picca_cont_set_forests(picca_ctx *ctx, int64_t num_forests, const int64_t *offsets,
+                           const double *flux, const double *ivar, const double *mean_cont,
+                           const double *x, const double *eta, const double *var_lss,
+                           const double *fudge, const int32_t *order);
+int picca_cont_fit(picca_ctx *ctx, const picca_cont_params *params, const double *a0,
+                   const double *b0, double *a, double *b, double *chi2, int32_t *ndata,
+                   int32_t *status, int32_t *evaluations, double *kernel_ms);
+
+/*
  * Diagnostics (no counterpart in picca): runs a plain device-to-device copy kernel over
  * `bytes` bytes (a multiple of 16) between two buffers of the context and returns its
  * device time.  Used to calibrate hardware byte counters (FETCH_SIZE / WRITE_SIZE) against

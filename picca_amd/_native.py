@@ -85,6 +85,8 @@ _PROTOTYPES = {
     "picca_xi_download": [_P, _P, _P, _P, _P, _P, _P],
     "picca_dmat_run": [_P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "picca_diag_copy": [_P, _I64, _P],
+    "picca_cont_set_forests": [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "picca_cont_fit": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "picca_neighbour_candidates": [_I64, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P,
                                    _P, _F64, _F64, _F64, _I32, _P, _P, _P, _P],
 }

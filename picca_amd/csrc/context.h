@@ -1,0 +1,163 @@
+// Device context of the C-ABI (shared by the entry points of the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/picca_hip.h"
+#include "engine.h"
+
+namespace picca_host {
+
+// message of the last failure on the calling thread (picca_last_error)
+std::string &last_error();
+
+inline int fail(int code, const std::string &msg) {
+    last_error() = msg;
+    return code;
+}
+
+#define PICCA_HIP_CHECK(expr)                                                              \
+    do {                                                                                   \
+        hipError_t err_ = (expr);                                                          \
+        if (err_ != hipSuccess)                                                            \
+            return picca_host::fail(PICCA_E_HIP,                                           \
+                                    std::string(#expr) + ": " + hipGetErrorString(err_));  \
+    } while (0)
+
+template <typename T>
+struct DeviceBuffer {
+    T *ptr = nullptr;
+    size_t count = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    hipError_t reserve(size_t n) {
+        if (n <= count && ptr) return hipSuccess;
+        release();
+        size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        hipError_t err = hipMalloc((void **)&ptr, bytes);
+        if (err != hipSuccess) {
+            ptr = nullptr;
+            return err;
+        }
+        count = n;
+        return hipSuccess;
+    }
+    hipError_t upload(const T *host, size_t n, hipStream_t stream) {
+        hipError_t err = reserve(n);
+        if (err != hipSuccess || n == 0) return err;
+        return hipMemcpyAsync(ptr, host, n * sizeof(T), hipMemcpyHostToDevice, stream);
+    }
+    // n values followed by `pad` zeros (kernels may read a few elements past the end)
+    hipError_t upload_padded(const T *host, size_t n, size_t pad, hipStream_t stream) {
+        hipError_t err = reserve(n + pad);
+        if (err != hipSuccess) return err;
+        if (n > 0) err = hipMemcpyAsync(ptr, host, n * sizeof(T), hipMemcpyHostToDevice, stream);
+        if (err != hipSuccess) return err;
+        return hipMemsetAsync(ptr + n, 0, pad * sizeof(T), stream);
+    }
+};
+
+// Runs body(begin, end) over [0, n) on up to hardware_concurrency threads.
+template <typename F>
+inline void parallel_for(int64_t n, F body) {
+    unsigned hw = std::thread::hardware_concurrency();
+    int64_t nthreads = std::max<int64_t>(1, std::min<int64_t>(hw ? hw : 1, 16));
+    if (n < 65536) nthreads = 1;
+    if (nthreads == 1) {
+        body(0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    int64_t step = (n + nthreads - 1) / nthreads;
+    for (int64_t t = 0; t < nthreads; ++t) {
+        int64_t lo = t * step, hi = std::min(n, lo + step);
+        if (lo >= hi) break;
+        pool.emplace_back([=] { body(lo, hi); });
+    }
+    for (auto &th : pool) th.join();
+}
+
+}  // namespace picca_host
+
+using picca_host::DeviceBuffer;
+using picca_host::fail;
+using picca_host::parallel_for;
+
+struct picca_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    int num_cus = 0;
+    int64_t lds_limit = 0;
+
+    // forest store
+    int64_t num_forests = 0;
+    int64_t num_pixels = 0;
+    double max_abs_r = 0;   // largest |r_comov| and |dist_m| of the store: bound the
+    double max_abs_d = 0;   // rounding of the pre-scaled coordinates of the fast loop
+    bool has_log_lambda = false;
+    bool has_order = false;
+    std::vector<int64_t> h_offsets;
+    DeviceBuffer<double> z, r, d, w, dw, ll, zq;
+    DeviceBuffer<picca::PixelRecord> pix;
+    DeviceBuffer<int64_t> off;
+    DeviceBuffer<int32_t> order;
+    DeviceBuffer<uint8_t> mono;
+
+    // xi pairs
+    int64_t num_pairs = 0;
+    int32_t num_rows = 0;
+    DeviceBuffer<int32_t> pa, pb;
+    DeviceBuffer<double> pang, pcs;
+    DeviceBuffer<uint8_t> pflags;
+    std::vector<int64_t> h_group_begin;
+    std::vector<int32_t> h_group_row;
+    DeviceBuffer<int64_t> gbegin;
+    DeviceBuffer<int32_t> grow;
+
+    // distortion-matrix inputs (stores given with log_lambda): per-forest sums of
+    // compute_dmat_forest_pairs_fast (py/picca/cf.py:577-594) and the pixel redshifts
+    std::vector<double> h_z;
+    DeviceBuffer<double> dll, sum_w, sw_dll, zfa, zfb;
+
+    // dmat batches
+    DeviceBuffer<int32_t> dpa, dpb;
+    DeviceBuffer<double> dang, dcs;
+    DeviceBuffer<uint8_t> dflags;
+    DeviceBuffer<double> ws_S, ws_eta;
+    DeviceBuffer<uint32_t> ws_U;
+    DeviceBuffer<int32_t> ws_band, ws_rng, ws_cnt;
+    DeviceBuffer<int64_t> ws_pix;
+    DeviceBuffer<unsigned long long> ws_touched, dstats;
+    DeviceBuffer<double> o_wdm, o_dmat, o_eff;
+
+    // continuum fit: resident spectra (forest pixels) and per-forest results
+    int64_t cont_forests = 0;
+    int64_t cont_pixels = 0;
+    DeviceBuffer<int64_t> c_off;
+    DeviceBuffer<double> c_flux, c_ivar, c_mean, c_x, c_eta, c_lss, c_fudge;
+    DeviceBuffer<int32_t> c_order;
+    DeviceBuffer<double> c_a0, c_b0, c_a, c_b, c_chi2;
+    DeviceBuffer<int32_t> c_ndata, c_status, c_evals;
+
+    // xi results
+    int64_t out_nb = 0;
+    int32_t out_rows = 0;
+    bool have_result = false;
+    DeviceBuffer<picca::XiBin> o_bins;
+    DeviceBuffer<unsigned long long> queue, stats;
+};
+

@@ -204,6 +204,38 @@ hipError_t launch_dmat_fill(const DmatConst &C, const DmatForests &F, const Dmat
                             hipStream_t stream);
 size_t dmat_fill_lds(const DmatConst &C);
 
+// ---------------------------------------------------------------------------------------
+// continuum fit
+// ---------------------------------------------------------------------------------------
+
+struct ContConst {
+    int32_t max_iter;
+    int32_t use_constant_weight;
+    double tol;   // stop when g^T H^-1 g / 2 < tol * max(1, |chi2|)
+};
+
+struct ContIn {
+    const int64_t *__restrict__ off;   // [n + 1]
+    const double *__restrict__ flux;
+    const double *__restrict__ ivar;
+    const double *__restrict__ mean_cont;   // mean continuum x transmission correction
+    const double *__restrict__ x;           // (log_lambda - Lmin) / (Lmax - Lmin)
+    const double *__restrict__ eta;
+    const double *__restrict__ var_lss;
+    const double *__restrict__ fudge;
+    const int32_t *__restrict__ order;      // per forest
+    const double *__restrict__ a0;          // per forest start
+    const double *__restrict__ b0;
+};
+
+struct ContOut {
+    double *a, *b, *chi2;
+    int32_t *ndata, *status, *evals;
+};
+
+hipError_t launch_cont_fit(const ContConst &K, const ContIn &I, const ContOut &O,
+                           int64_t n_forests, hipStream_t stream);
+
 hipError_t launch_xi(int mode, bool xcorr, bool general, bool lds_hist, const XiConst &C,
                      const ForestView &F, const PairView &P, const GroupView &G,
                      const XiOut &O, int grid, size_t lds, hipStream_t stream);
