@@ -180,6 +180,31 @@ int picca_cont_fit(picca_ctx *ctx, const picca_cont_params *params, const double
                    int32_t *status, int32_t *evaluations, double *kernel_ms);
 
 /*
+ * Seed network of the continuum fit (no counterpart in picca: it predicts the start
+ * (a, b) of the fit above from the binned spectrum).  A two-layer perceptron
+ * y = relu(x W1 + b1) W2 + b2 with n_in inputs (multiple of 32, <= 256), n_hidden units
+ * (multiple of 16, <= 128) and 2 outputs; W1 is [n_in][n_hidden] row-major, W2
+ * [n_hidden][2].  The n_in x n_hidden products run in BF16 on the matrix cores with FP32
+ * accumulation; parameters, gradients and Adam state are FP32.
+ *
+ * picca_mlp_init sets the parameters (and resets Adam); picca_mlp_get reads them back;
+ * picca_mlp_forward evaluates n rows of x [n][n_in] into y [n][2]; picca_mlp_gradients
+ * returns the gradients of loss = sum (y - t)^2 / (2 n) over one batch without updating;
+ * picca_mlp_set_data makes a training set resident and picca_mlp_train runs `steps`
+ * Adam steps (beta1 0.9, beta2 0.999, eps 1e-8) on minibatches of `batch` rows drawn
+ * without replacement per epoch from a seeded permutation, writing each step's loss.
+ */
+int picca_mlp_init(picca_ctx *ctx, int32_t n_in, int32_t n_hidden, const float *w1,
+                   const float *b1, const float *w2, const float *b2);
+int picca_mlp_get(picca_ctx *ctx, float *w1, float *b1, float *w2, float *b2);
+int picca_mlp_forward(picca_ctx *ctx, int64_t n, const float *x, float *y);
+int picca_mlp_gradients(picca_ctx *ctx, int64_t n, const float *x, const float *t,
+                        float *g_w1, float *g_b1, float *g_w2, float *g_b2, float *loss);
+int picca_mlp_set_data(picca_ctx *ctx, int64_t n, const float *x, const float *t);
+int picca_mlp_train(picca_ctx *ctx, int64_t steps, int32_t batch, float lr, uint64_t seed,
+                    float *losses);
+
+/*
  * Diagnostics (no counterpart in picca): runs a plain device-to-device copy kernel over
  * `bytes` bytes (a multiple of 16) between two buffers of the context and returns its
  * device time.  Used to calibrate hardware byte counters (FETCH_SIZE / WRITE_SIZE) against

@@ -73,6 +73,12 @@ _I64 = ctypes.c_int64
 _F64 = ctypes.c_double
 
 # name -> argument types (every function returns int)
+class ContParams(ctypes.Structure):
+    """``picca_cont_params``."""
+    _fields_ = [("max_iter", ctypes.c_int32), ("use_constant_weight", ctypes.c_int32),
+                ("tol", ctypes.c_double)]
+
+
 _PROTOTYPES = {
     "picca_abi_version": [],
     "picca_device_count": [_P],
@@ -87,6 +93,12 @@ _PROTOTYPES = {
     "picca_diag_copy": [_P, _I64, _P],
     "picca_cont_set_forests": [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "picca_cont_fit": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "picca_mlp_init": [_P, _I32, _I32, _P, _P, _P, _P],
+    "picca_mlp_get": [_P, _P, _P, _P, _P],
+    "picca_mlp_forward": [_P, _I64, _P, _P],
+    "picca_mlp_gradients": [_P, _I64, _P, _P, _P, _P, _P, _P, _P],
+    "picca_mlp_set_data": [_P, _I64, _P, _P],
+    "picca_mlp_train": [_P, _I64, _I32, ctypes.c_float, ctypes.c_uint64, _P],
     "picca_neighbour_candidates": [_I64, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P,
                                    _P, _F64, _F64, _F64, _I32, _P, _P, _P, _P],
 }

@@ -17,7 +17,8 @@
 // stops when the Newton decrement g^T H^-1 g / 2 (Migrad's EDM) falls below tol * max(1,
 // |chi2|).  The slope stays at 0 for order-0 forests (py/picca/delta_extraction/
 // expected_fluxes/utils.py:109).  The validity checks of compute_continuum (:111-120:
-// negative or all-zero continuum) run on the final parameters.
+// not converged, then negative, then all-zero continuum, each overriding the previous)
+// run on the final parameters.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -166,9 +167,10 @@ __global__ __launch_bounds__(256) void cont_fit_kernel(ContConst K, ContIn I, Co
     negative = __ballot(negative) != 0;
     nonzero = __ballot(nonzero) != 0;
     for (int off = 32; off > 0; off >>= 1) ndata += __shfl_xor(ndata, off, 64);
+    // the reference's checks in its order, later ones overriding earlier ones
     if (status == 0 && !(s.chi == s.chi)) status = 4;
-    if (status == 0 && negative) status = 2;
-    if (status == 0 && !nonzero) status = 3;
+    if (negative) status = 2;
+    if (!nonzero) status = 3;
     if (lane == 0) {
         O.a[f] = a;
         O.b[f] = b;

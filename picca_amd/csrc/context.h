@@ -153,6 +153,17 @@ struct picca_ctx {
     DeviceBuffer<double> c_a0, c_b0, c_a, c_b, c_chi2;
     DeviceBuffer<int32_t> c_ndata, c_status, c_evals;
 
+    // seed network: parameters, Adam state, training data and scratch
+    int32_t mlp_in = 0, mlp_hidden = 0;
+    int64_t mlp_step = 0;   // Adam time step
+    DeviceBuffer<float> m_w1t, m_b1, m_w2, m_b2;
+    DeviceBuffer<uint16_t> m_w1t_bf16;
+    DeviceBuffer<float> g_w1t, g_b1, g_w2, g_b2, g_loss;
+    DeviceBuffer<float> a_m, a_v;
+    DeviceBuffer<float> m_x, m_t, m_h, m_y, m_dh;
+    DeviceBuffer<int32_t> m_rows;
+    int64_t mlp_rows = 0;   // rows of the resident training data
+
     // xi results
     int64_t out_nb = 0;
     int32_t out_rows = 0;

@@ -236,6 +236,43 @@ struct ContOut {
 hipError_t launch_cont_fit(const ContConst &K, const ContIn &I, const ContOut &O,
                            int64_t n_forests, hipStream_t stream);
 
+// ---------------------------------------------------------------------------------------
+// seed network of the continuum fit
+// ---------------------------------------------------------------------------------------
+
+constexpr int MLP_MAX_IN = 256;        // inputs (multiple of 32)
+constexpr int MLP_MAX_HIDDEN = 128;    // hidden units (multiple of 16)
+constexpr int MLP_MAX_TILES_PER_WAVE = (MLP_MAX_IN / 16) * (MLP_MAX_HIDDEN / 16) / 4;
+constexpr int MLP_BW_ROWS = 256;       // batch rows per block of the dW1 product
+
+struct MlpView {
+    int32_t n_in, n_hidden;
+    float *w1t;          // [n_hidden][n_in] FP32 master (W1 transposed)
+    uint16_t *w1t_bf16;  // [n_hidden][n_in] BF16 copy read by the matrix cores
+    float *b1;           // [n_hidden]
+    float *w2;           // [n_hidden][2]
+    float *b2;           // [2]
+};
+
+struct MlpGrads {
+    float *w1t, *b1, *w2, *b2;
+    float *loss;
+};
+
+struct MlpAdam {
+    float *m, *v;        // [n_hidden * n_in + 3 * n_hidden + 2], parameter order w1t b1 w2 b2
+};
+
+hipError_t launch_mlp_forward(const MlpView &M, const float *x, const int32_t *rows, int64_t n,
+                              float *h, float *y, hipStream_t stream);
+hipError_t launch_mlp_backward(const MlpView &M, const MlpGrads &G, const float *x,
+                               const float *target, const int32_t *rows, int64_t n,
+                               const float *h, const float *y, float *dh, hipStream_t stream);
+hipError_t launch_mlp_adam(const MlpView &M, const MlpGrads &G, const MlpAdam &A, float lr,
+                           float beta1, float beta2, float eps, float corr1, float corr2,
+                           hipStream_t stream);
+hipError_t launch_mlp_pack(const MlpView &M, hipStream_t stream);
+
 hipError_t launch_xi(int mode, bool xcorr, bool general, bool lds_hist, const XiConst &C,
                      const ForestView &F, const PairView &P, const GroupView &G,
                      const XiOut &O, int grid, size_t lds, hipStream_t stream);

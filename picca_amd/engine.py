@@ -158,6 +158,37 @@ class Engine:
         (we, xi, rp, rt, z), nb = self.xi_download(params)
         return XiResult(we, xi, rp, rt, z, nb, report.as_dict())
 
+    # -- continuum fit --------------------------------------------------------
+    def cont_set_forests(self, offsets, flux, ivar, mean_cont, x, eta, var_lss, fudge, order):
+        offsets = as_i64(offsets)
+        arrays = [as_f64(a) for a in (flux, ivar, mean_cont, x, eta, var_lss, fudge)]
+        order = as_i32(order)
+        check(self._lib.picca_cont_set_forests(self._ctx, len(offsets) - 1, ptr(offsets),
+                                               *(ptr(a) for a in arrays), ptr(order)),
+              "picca_cont_set_forests")
+        self._cont_forests = len(offsets) - 1
+
+    def cont_fit(self, a0, b0, max_iter=100, tol=1e-12, use_constant_weight=False):
+        """Fit every resident forest from (a0, b0).  Returns a dict of arrays a, b, chi2,
+        ndata, status, evaluations and the kernel time kernel_ms."""
+        n = self._cont_forests
+        a0 = as_f64(a0)
+        b0 = as_f64(b0)
+        if len(a0) != n or len(b0) != n:
+            raise ValueError("one start value per forest is needed")
+        params = _native.ContParams(int(max_iter), int(bool(use_constant_weight)), float(tol))
+        out = {"a": np.zeros(n), "b": np.zeros(n), "chi2": np.zeros(n),
+               "ndata": np.zeros(n, np.int32), "status": np.zeros(n, np.int32),
+               "evaluations": np.zeros(n, np.int32)}
+        ms = ctypes.c_double()
+        check(self._lib.picca_cont_fit(self._ctx, ctypes.byref(params), ptr(a0), ptr(b0),
+                                       ptr(out["a"]), ptr(out["b"]), ptr(out["chi2"]),
+                                       ptr(out["ndata"]), ptr(out["status"]),
+                                       ptr(out["evaluations"]), ctypes.byref(ms)),
+              "picca_cont_fit")
+        out["kernel_ms"] = ms.value
+        return out
+
     def diag_copy(self, nbytes):
         """Plain device copy of nbytes (multiple of 16); returns the kernel time in ms."""
         ms = ctypes.c_double()
