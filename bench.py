@@ -132,13 +132,30 @@ def main():
     parser.add_argument("--traffic-bytes", type=float, default=-1.0,
                         help="HBM bytes per launch of the pair kernel (default: the "
                              "counter-derived figure in profiles/ for this workload)")
+    parser.add_argument("--mode", default="xi", choices=("xi", "dmat", "continuum", "precision"),
+                        help="xi: the pair kernel (default); dmat: the distortion matrix; "
+                             "continuum: plain vs learned-seed continuum fits; precision: "
+                             "xi error and edge pairs on the heavy-tailed survey")
+    parser.add_argument("--reject", type=float, default=0.99,
+                        help="dmat: fraction of neighbour pairs rejected")
+    parser.add_argument("--spectra", type=int, default=200000,
+                        help="continuum: spectra fitted (and as many more for training)")
     args = parser.parse_args()
 
     if args.gpus != 1:
         print("bench.py measures one GPU only: this engine has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "dmat":
+        return bench_dmat(args)
+    if args.mode == "continuum":
+        return bench_continuum(args)
+    if args.mode == "precision":
+        return bench_precision(args)
+    return bench_xi(args)
 
+
+def bench_xi(args):
     from picca_amd import batch, synth
     from picca_amd.constants import Cosmo
     from picca_amd.engine import device_info, get_engine, params_from_globals
@@ -286,6 +303,234 @@ def main():
         result["speedup_vs_cpu"] = value / cpu_rate
     print(json.dumps(result))
 
+
+
+class DmatGlobals(Globals):
+    num_model_bins_r_par = 100
+    num_model_bins_r_trans = 50
+
+
+def bench_dmat(args):
+    """Distortion matrix of the 100k-forest survey: neighbour pairs kept with probability
+    1 - reject (one seeded legacy-numpy draw per pair, in pair-list order), 50x50 data bins,
+    100x50 model bins, redshift evolution on."""
+    from picca_amd import batch, synth
+    from picca_amd.constants import Cosmo
+    from picca_amd.engine import device_info, get_engine, params_from_globals
+    from picca_amd.utils import compute_ang_max
+
+    survey = synth.make_survey("survey_100k", seed=args.seed, scale=args.scale)
+    store = survey.store
+    ang_max = compute_ang_max(Cosmo(0.315, verbose=False), Globals.r_trans_max, survey.z_min)
+    pairs = batch.xi_pairs(store, survey.healpix, ang_max)
+    keep = np.random.RandomState(args.seed).rand(pairs.num_pairs) > args.reject
+    a, b = pairs.pair_a[keep], pairs.pair_b[keep]
+    ang, flags = pairs.pair_ang[keep], pairs.pair_flags[keep]
+    log("dmat: %d of %d neighbour pairs kept" % (len(a), pairs.num_pairs))
+    params = params_from_globals(DmatGlobals)
+    eng = get_engine(0)
+    info = device_info(0)
+    eng.set_forests(store, key="bench_dmat")
+    for w in range(args.warmup):
+        eng.dmat(params, a, b, ang, flags)
+    times = []
+    out = None
+    for k in range(args.steps):
+        t = time.perf_counter()
+        out = eng.dmat(params, a, b, ang, flags)
+        times.append(time.perf_counter() - t)
+        log("dmat step %d: %.2f s" % (k, times[-1]))
+    sec = float(np.mean(times))
+    stats = out[6]
+    result = {
+        "metric": "distortion-matrix in-range pixel pairs/s (1 GPU)",
+        "value": stats[0] / sec, "unit": "pixel pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * sec, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "dmat_100k", "scale": args.scale, "forests": store.num_forests,
+                   "neighbour_pairs": pairs.num_pairs, "kept_pairs": int(len(a)),
+                   "reject": args.reject, "in_range_pairs": int(stats[0]),
+                   "selected_pairs": int(stats[1]), "bins": "50x50", "model_bins": "100x50",
+                   "device": info["name"], "timing": "wall time of picca_dmat_run, "
+                   "including the 100 MB download of the matrix"},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        from oracle import cf_oracle as O
+        rng = np.random.default_rng(1)
+        sel = np.sort(rng.choice(len(a), min(len(a), 12), replace=False))
+        oparams = O.params_from_module(DmatGlobals)
+        tc = time.time()
+        ref, ref_stats = O.dmat_rows(oparams, store, a[sel], b[sel], ang[sel], flags[sel])
+        cpu_s = time.time() - tc
+        mine = eng.dmat(params, a[sel], b[sel], ang[sel], flags[sel])
+        scale = np.abs(ref[1]).max()
+        err = float(np.abs(mine[1] - ref[1]).max() / scale)
+        result["cpu_baseline"] = {
+            "value": ref_stats[0] / cpu_s, "unit": "pixel pairs/s", "cores": 1, "kind": "port",
+            "sample": "%d random kept pairs (seed 1), %d in-range pixel pairs, FP64 C oracle "
+                      "(scalar restatement of compute_dmat_forest_pairs_fast)"
+                      % (len(sel), int(ref_stats[0])), "seconds": cpu_s}
+        result["parity_sample"] = {"dmat_max_abs_err_over_max": err,
+                                   "counts_equal": bool(list(mine[6]) == list(ref_stats))}
+    print(json.dumps(result))
+
+
+def continuum_sets(n, seed, chunk=20000):
+    """Forest inputs, binned network inputs and scales of n synthetic spectra, made in
+    chunks to bound host memory."""
+    from picca_amd import synth_spectra as S
+    parts = []
+    for k0 in range(0, n, chunk):
+        spectra = S.make_spectra(min(chunk, n - k0), seed=seed * 1000003 + k0)
+        inputs = S.forest_inputs(spectra)
+        x, scale = S.binned_inputs(spectra)
+        parts.append((inputs, x, scale))
+    offsets = [np.zeros(1, np.int64)]
+    base = 0
+    for inputs, _, _ in parts:
+        offsets.append(inputs["offsets"][1:] + base)
+        base += inputs["offsets"][-1]
+    merged = {"offsets": np.concatenate(offsets)}
+    for name in ("log_lambda", "flux", "ivar", "mean_cont", "x", "eta", "var_lss", "fudge",
+                 "log_lambda_min", "log_lambda_max", "z_qso"):
+        merged[name] = np.concatenate([p[0][name] for p in parts])
+    return merged, np.concatenate([p[1] for p in parts]), np.concatenate([p[2] for p in parts])
+
+
+def bench_continuum(args):
+    """Continuum fits of picca's first iteration (mean continuum 1, eta 1, var_lss 0.1) on
+    synthetic spectra: the plain GPU fit (Migrad's start) against the fit started from the
+    seed network's prediction, which is trained on the plain fit of another set."""
+    from picca_amd import continuum, seed_mlp
+    from picca_amd.engine import device_info
+
+    n = args.spectra
+    t0 = time.time()
+    train_in, train_x, train_scale = continuum_sets(n // 2, seed=args.seed + 1)
+    test_in, test_x, test_scale = continuum_sets(n, seed=args.seed)
+    log("continuum: %d training and %d test spectra, %d + %d forest pixels (%.1f s)"
+        % (n // 2, n, train_in["offsets"][-1], test_in["offsets"][-1], time.time() - t0))
+    info = device_info(0)
+    train_fit = continuum.fit_arrays(train_in, order=1)
+    ok = train_fit["status"] == 0
+    net = seed_mlp.SeedNetwork(seed=args.seed)
+    t1 = time.time()
+    losses = net.train(train_x[ok], seed_mlp.targets(train_fit["a"][ok], train_fit["b"][ok],
+                                                    train_scale[ok]),
+                       epochs=30, batch=512, lr=1e-3)
+    train_s = time.time() - t1
+    log("seed network: %d Adam steps in %.1f s, loss %.3e -> %.3e"
+        % (len(losses), train_s, losses[:20].mean(), losses[-20:].mean()))
+    plain_ms, seeded_ms, infer_s = [], [], []
+    plain = seeded = None
+    for k in range(args.warmup + args.steps):
+        plain = continuum.fit_arrays(test_in, order=1)
+        t = time.perf_counter()
+        y = net.predict(test_x)
+        infer = time.perf_counter() - t
+        a0, b0 = seed_mlp.seeds(y, test_scale)
+        seeded = continuum.fit_arrays(test_in, order=1, a0=a0, b0=b0)
+        if k >= args.warmup:
+            plain_ms.append(plain["kernel_ms"])
+            seeded_ms.append(seeded["kernel_ms"])
+            infer_s.append(infer)
+        log("step %d: plain fit %.2f ms, seeded fit %.2f ms, inference %.1f ms (wall)"
+            % (k, plain["kernel_ms"], seeded["kernel_ms"], 1e3 * infer))
+    both = (plain["status"] == 0) & (seeded["status"] == 0)
+    dchi = np.abs(seeded["chi2"][both] - plain["chi2"][both])
+    p_ms, s_ms = float(np.mean(plain_ms)), float(np.mean(seeded_ms))
+    result = {
+        "metric": "continuum fits/s (1 GPU), learned seed vs plain fit",
+        "value": n / (s_ms / 1e3), "unit": "fits/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": s_ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 fit, bf16/fp32 network",
+        "data": "synthetic",
+        "config": {"workload": "continuum_%d" % n, "spectra": n, "training_spectra": n // 2,
+                   "forest_pixels": int(test_in["offsets"][-1]),
+                   "plain_fit_ms": p_ms, "seeded_fit_ms": s_ms,
+                   "plain_fits_per_s": n / (p_ms / 1e3),
+                   "seeded_speedup_fit_kernel": p_ms / s_ms,
+                   "inference_wall_ms": 1e3 * float(np.mean(infer_s)),
+                   "plain_mean_evaluations": float(plain["evaluations"].mean()),
+                   "seeded_mean_evaluations": float(seeded["evaluations"].mean()),
+                   "fits_valid_both": int(both.sum()),
+                   "chi2_max_abs_diff": float(dchi.max()) if dchi.size else 0.0,
+                   "chi2_frac_within_1e-4": float((dchi < 1e-4).mean()) if dchi.size else 1.0,
+                   "training_steps": int(len(losses)), "training_s": train_s,
+                   "device": info["name"]},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        from oracle import cont_oracle as C
+        from picca_amd import synth_spectra as S
+        m = min(n, 300)
+        tc = time.time()
+        res = [C.fit(S.forest_spec(test_in, k), order=1) for k in range(m)]
+        cpu_s = time.time() - tc
+        ref = np.array([r[2] for r in res])
+        rel = np.abs(plain["chi2"][:m] - ref) / np.abs(ref)
+        result["cpu_baseline"] = {
+            "value": m / cpu_s, "unit": "fits/s", "cores": 1, "kind": "port",
+            "sample": "first %d test spectra, FP64 numpy objective minimised by scipy "
+                      "Nelder-Mead + Newton polish (iminuit is not installed)" % m,
+            "seconds": cpu_s}
+        result["parity_sample"] = {"chi2_max_rel_err_vs_oracle": float(rel.max())}
+    print(json.dumps(result))
+
+
+def bench_precision(args):
+    """xi of the heavy-tailed survey (delta = 0.3 t(3), w log-uniform in [1e-3, 1e2]) on the
+    GPU against the FP64 oracle on a sample of its forest pairs: per-bin relative error and
+    edge pairs."""
+    args.workload = "survey_100k_heavy"
+    args.no_cpu = True
+    from picca_amd import batch, synth
+    from picca_amd.constants import Cosmo
+    from picca_amd.engine import get_engine, params_from_globals
+    from picca_amd.utils import compute_ang_max
+    from oracle import cf_oracle as O
+
+    survey = synth.make_survey(args.workload, seed=args.seed, scale=args.scale)
+    store = survey.store
+    ang_max = compute_ang_max(Cosmo(0.315, verbose=False), Globals.r_trans_max, survey.z_min)
+    pairs = batch.xi_pairs(store, survey.healpix, ang_max)
+    params = params_from_globals(Globals)
+    eng = get_engine(0)
+    eng.set_forests(store, key="bench_precision")
+    eng.set_xi_pairs(pairs.pair_a, pairs.pair_b, pairs.pair_ang, pairs.pair_flags,
+                     pairs.pair_row, pairs.num_rows, key="bench_precision")
+    full = eng.xi(params)
+    sample = sample_pairs(pairs, store, args.cpu_pairs)
+    out, stats = O.xi_rows(O.params_from_module(Globals), store, sample.pair_a, sample.pair_b,
+                           sample.pair_ang, sample.pair_flags, sample.pair_row,
+                           sample.num_rows, num_threads=cpu_cores())
+    eng.set_xi_pairs(sample.pair_a, sample.pair_b, sample.pair_ang, sample.pair_flags,
+                     sample.pair_row, sample.num_rows, key="sample")
+    g = eng.xi(params)
+    mask = out[0] > 0
+    xi_g = g.xi[mask] / g.weights[mask]
+    xi_o = out[1][mask] / out[0][mask]
+    rel = np.abs(xi_g - xi_o) / np.abs(xi_o)
+    wrel = np.abs(g.weights[mask] - out[0][mask]) / out[0][mask]
+    result = {
+        "metric": "xi relative error vs FP64 oracle (heavy-tailed delta, w in [1e-3, 1e2])",
+        "value": float(rel.max()), "unit": "relative error", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": full.report["kernel_ms"], "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "scale": args.scale,
+                   "forest_pairs": pairs.num_pairs, "sample_forest_pairs": sample.num_pairs,
+                   "sample_bins": int(mask.sum()),
+                   "xi_rel_err_median": float(np.median(rel)),
+                   "xi_rel_err_p99": float(np.quantile(rel, 0.99)),
+                   "weights_rel_err_max": float(wrel.max()),
+                   "num_pairs_mismatches": int(np.count_nonzero(g.num_pairs != out[5])),
+                   "edge_pairs_full": int(full.report["edge_pairs"]),
+                   "edge_pairs_sample_gpu": int(g.report["edge_pairs"]),
+                   "edge_pairs_sample_oracle": int(stats[1]),
+                   "overflow_pairs_full": int(full.report["overflow_pairs"])},
+    }
+    print(json.dumps(result))
 
 if __name__ == "__main__":
     main()

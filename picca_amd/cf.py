@@ -95,7 +95,7 @@ def current_params():
 # ---------------------------------------------------------------------------
 
 class _Catalogue:
-    def __init__(self, data_1, data_2):
+    def __init__(self, data_1, data_2, angular=False):
         self.healpixs1 = sorted(data_1)
         deltas1 = [d for hp in self.healpixs1 for d in data_1[hp]]
         deltas2 = []
@@ -104,7 +104,7 @@ class _Catalogue:
             deltas2 = [d for hp in self.healpixs2 for d in data_2[hp]]
         self.deltas = deltas1 + deltas2
         self.num1 = len(deltas1)
-        self.store = ForestStore.from_deltas(self.deltas)
+        self.store = ForestStore.from_deltas(self.deltas, angular=angular)
         self.index = {id(d): k for k, d in enumerate(self.deltas)}
         # targets in the reference's neighbour order: healpix ascending, then list order
         if data_2 is None:
@@ -117,7 +117,7 @@ class _Catalogue:
         for hp in self.healpixs1:
             self.range1[hp] = (start, start + len(data_1[hp]))
             start += len(data_1[hp])
-        self.key = (id(data_1), id(data_2), len(self.deltas))
+        self.key = (id(data_1), id(data_2), bool(angular), len(self.deltas))
 
     def index_of(self, delta):
         try:
@@ -133,9 +133,9 @@ def _catalogue():
     global _catalogue_cache
     if data is None:
         raise RuntimeError("cf.data is not set")
-    key = (id(data), id(data2))
-    if _catalogue_cache is None or _catalogue_cache.key[:2] != key:
-        _catalogue_cache = _Catalogue(data, data2)
+    key = (id(data), id(data2), bool(ang_correlation))
+    if _catalogue_cache is None or _catalogue_cache.key[:3] != key:
+        _catalogue_cache = _Catalogue(data, data2, angular=bool(ang_correlation))
     return _catalogue_cache
 
 

@@ -78,8 +78,12 @@ class ForestStore:
                 if getattr(self, name) is not None}
 
     @classmethod
-    def from_deltas(cls, deltas):
-        """Pack a sequence of Delta objects (after read_deltas) in the given order."""
+    def from_deltas(cls, deltas, angular=False):
+        """Pack a sequence of Delta objects (after read_deltas) in the given order.
+
+        With ``angular`` the separation coordinates r_comov and dist_m are both the
+        wavelength 10**log_lambda, which is what compute_xi hands its kernel for the
+        angular correlation (py/picca/cf.py:186-191)."""
         deltas = list(deltas)
         lengths = np.array([len(d.z) for d in deltas], dtype=np.int64)
         offsets = np.zeros(len(deltas) + 1, dtype=np.int64)
@@ -104,7 +108,13 @@ class ForestStore:
         orders = np.array([-1 if d.order is None else int(d.order) for d in deltas],
                           dtype=np.int32)
         has_ll = all(d.log_lambda is not None for d in deltas)
-        return cls(offsets, cat("z"), cat("r_comov"), cat("dist_m"), cat("weights"),
+        if angular:
+            wave = (np.concatenate([10.0 ** np.asarray(d.log_lambda, dtype=np.float64)
+                                    for d in deltas]) if deltas else np.zeros(0))
+            r_comov, dist_m = wave, wave
+        else:
+            r_comov, dist_m = cat("r_comov"), cat("dist_m")
+        return cls(offsets, cat("z"), r_comov, dist_m, cat("weights"),
                    cat("delta"),
                    np.array([d.z_qso for d in deltas], dtype=np.float64),
                    np.array([d.ra for d in deltas], dtype=np.float64),

@@ -110,6 +110,39 @@ class Setup:
             self.ang_max = utils.compute_ang_max(cosmo, self.r_trans_max, z_min, z_min2)
 
 
+CF_ANGL_CASE = ["--in-dir", DELTA_LYA, "--nproc", "1", "--in-attributes", ATTRIBUTES]
+
+
+class AngularSetup:
+    """Globals + data of picca's test_cf_angl (py/picca/tests/test_3_cor.py:206-227) as
+    picca_cf_angl sets them (py/picca/bin/picca_cf_angl.py:172-233): wavelength ratio in
+    [1, 1.1) for r_par, angle below 0.02 rad for r_trans, deltas read without cosmology."""
+
+    def __init__(self):
+        self.r_par_min, self.r_par_max = 1.0, 1.1
+        self.r_trans_max = 0.02
+        self.ang_max = 0.02
+        self.num_bins_r_par = 50
+        self.num_bins_r_trans = 50
+        self.num_model_bins_r_par = None
+        self.num_model_bins_r_trans = None
+        self.z_min_pairs = self.z_max_pairs = None
+        self.zerr_cut_deg = self.zerr_cut_kms = None
+        self.z_ref = 2.25
+        self.alpha = 2.9
+        self.alpha2 = None
+        self.reject = 1.0
+        self.x_correlation = False
+        self.rmu_binning = False
+        self.ang_correlation = True
+        self.remove_same_half_plate_close_pairs = False
+        self.redshift_evolution_in_distortion_matrix = True
+        self.data, self.num_data, _, _ = io.read_deltas(
+            DELTA_LYA, 16, constants.ABSORBER_IGM["LYA"], 2.9, 2.25, None,
+            delta_attributes=ATTRIBUTES)
+        self.data2 = None
+
+
 def reduce_rows(rows):
     """The parent-process reduction of py/picca/bin/picca_cf.py:466-484."""
     rows = np.array(rows)

@@ -66,6 +66,23 @@ def test_fixture_dmat_matches_oracle_and_golden(case):
     cor_cases.compare_to_golden(cols, cor_cases.golden_columns(case))
 
 
+@pytest.mark.parametrize("case", sorted(cor_cases.DMAT_CASES))
+def test_dmat_driver_reproduces_golden(case, tmp_path):
+    from picca_amd.bin import picca_dmat
+    out = str(tmp_path / (case + ".fits.gz"))
+    picca_dmat.main(cor_cases.DMAT_CASES[case] + ["--out", out])
+    mine = cor_cases.FITS(out)
+    golden = cor_cases.FITS(cor_cases.golden_path(case))
+    for key in ("NPALL", "NPUSED"):
+        assert mine["DMAT"].read_header()[key] == golden["DMAT"].read_header()[key]
+    result = {}
+    for hdu in mine.hdus[1:]:
+        assert hdu.get_colnames() == golden[hdu.get_extname()].get_colnames()
+        for col in hdu.get_colnames():
+            result[col] = hdu[col][:]
+    cor_cases.compare_to_golden(result, cor_cases.golden_columns(case))
+
+
 class Globals:
     num_bins_r_par = 15
     num_bins_r_trans = 15

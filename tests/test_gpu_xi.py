@@ -85,6 +85,18 @@ def test_driver_reproduces_golden(case, tmp_path):
     cor_cases.compare_to_golden(result, cor_cases.golden_columns(case))
 
 
+def test_angular_driver_reproduces_golden(tmp_path):
+    from picca_amd.bin import picca_cf_angl
+    out = str(tmp_path / "cf_angl.fits.gz")
+    picca_cf_angl.main(cor_cases.CF_ANGL_CASE + ["--out", out])
+    mine = FITS(out)
+    result = {}
+    for hdu in mine.hdus[1:]:
+        for col in hdu.get_colnames():
+            result[col] = hdu[col][:]
+    cor_cases.compare_to_golden(result, cor_cases.golden_columns("cf_angl"))
+
+
 def test_compute_xi_single_call_matches_oracle():
     setup = cor_cases.Setup(cor_cases.CF_CASES["cf"])
     set_globals(setup)

@@ -15,7 +15,7 @@ from oracle import cf_oracle as O
 _NEIGHBOURS = {}
 
 
-def oracle_rows(setup):
+def oracle_rows(setup, ang_correlation=False):
     params = O.params_from_module(setup)
     key = (id(setup.data), id(setup.data2), setup.ang_max)
     neigh = _NEIGHBOURS.get(key)
@@ -26,7 +26,7 @@ def oracle_rows(setup):
     rows = []
     stats = np.zeros(3, dtype=np.int64)
     for hp in sorted(setup.data):
-        res, st = O.compute_xi(params, setup.data, [hp], neigh)
+        res, st = O.compute_xi(params, setup.data, [hp], neigh, ang_correlation)
         rows.append(res)
         stats += st
     return rows, stats
@@ -60,3 +60,13 @@ def test_known_answers():
     assert cor_cases.golden_columns("cf_zcuts")["NB"].sum() == 153260
     assert cor_cases.golden_columns("cf_image")["NB"].sum() == 21661136
     assert list(cor_cases.golden_columns("cf_image")["HEALPID"]) == [16, 69, 94, 711]
+
+
+def test_oracle_reproduces_golden_angular():
+    setup = cor_cases.AngularSetup()
+    rows, stats = oracle_rows(setup, ang_correlation=True)
+    result = cor_cases.reduce_rows(rows)
+    result["HEALPID"] = np.array(sorted(setup.data))
+    golden = cor_cases.golden_columns("cf_angl")
+    cor_cases.compare_to_golden(result, golden)
+    assert stats[0] == golden["NB"].sum() > 0
