@@ -130,6 +130,21 @@ int picca_xi_download(picca_ctx *ctx, double *weights, double *xi, double *r_par
                       double *r_trans, double *z, int64_t *num_pairs);
 
 /*
+ * Forest x quasar cross-correlation (compute_xi of py/picca/xcf.py:131-220 with its
+ * per-pair kernel :223-322).  Pair k correlates forest pair_forest[k] of the resident
+ * store (the deltas) with a quasar at r_comov obj_r[k], dist_m obj_d[k], redshift obj_z[k]
+ * and weight obj_w[k], at angle pair_ang[k], into output row pair_row[k]; the pairs of one
+ * forest must be contiguous and belong to one row, rows contiguous.  In the angular
+ * mode obj_r is the quasar's wavelength (obj_d unused).  Results are read with
+ * picca_xi_download.  cos/sin(ang/2) are taken with the host C library.
+ */
+int picca_xcf_set_pairs(picca_ctx *ctx, int64_t num_pairs, const int32_t *pair_forest,
+                        const double *obj_r, const double *obj_d, const double *obj_z,
+                        const double *obj_w, const double *pair_ang, const int32_t *pair_row,
+                        int32_t num_rows, int32_t angular);
+int picca_xcf_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_report *report);
+
+/*
  * Distortion matrix (compute_dmat, py/picca/cf.py:390-517) over an explicit list of
  * kept forest pairs (the rejection draws of py/picca/cf.py:444 are made by the caller,
  * in reference order).  Needs a forest store with log_lambda and order.

@@ -153,6 +153,76 @@ void oracle_xi_pair(const oracle_params *p,
 }
 
 /*
+ * compute_xi_forest_pairs_fast of the forest x quasar cross-correlation
+ * (py/picca/xcf.py:227-322): forest pixels i against quasars j, each quasar with its own
+ * angle.  r_par keeps its sign; the lower range limit is exclusive (r_par <= r_par_min is
+ * rejected).  Bin indices >= np*nt (rounding at the range limit) are counted in stats[2]
+ * and left out; stats[0] counts accepted pairs, stats[1] edge pairs.
+ */
+void oracle_xcf_pair(const oracle_params *p,
+                     const double *z1, const double *r_comov1, const double *dist_m1,
+                     const double *weights1, const double *delta1, int64_t n1,
+                     const double *z2, const double *r_comov2, const double *dist_m2,
+                     const double *weights2, const double *ang, int64_t n2,
+                     double *rebin_weight, double *rebin_xi, double *rebin_r_par,
+                     double *rebin_r_trans, double *rebin_z, int64_t *rebin_num_pairs,
+                     int64_t *stats) {
+    const int64_t nbins = (int64_t)p->num_bins_r_par * p->num_bins_r_trans;
+    const double width_p = (p->r_par_max - p->r_par_min) / p->num_bins_r_par;
+    const double width_t = p->r_trans_max / p->num_bins_r_trans;
+    for (int64_t i = 0; i < n1; ++i) {
+        if (weights1[i] == 0) continue;
+        for (int64_t j = 0; j < n2; ++j) {
+            if (weights2[j] == 0) continue;
+            double z = (z1[i] + z2[j]) / 2;
+            if ((p->has_z_min_pairs && z < p->z_min_pairs) ||
+                (p->has_z_max_pairs && z > p->z_max_pairs))
+                continue;
+            double r_par, r_trans;
+            if (p->ang_correlation) {
+                r_par = r_comov1[i] / r_comov2[j];
+                r_trans = ang[j];
+            } else {
+                r_par = (r_comov1[i] - r_comov2[j]) * cos(ang[j] / 2);
+                r_trans = (dist_m1[i] + dist_m2[j]) * sin(ang[j] / 2);
+            }
+            if (p->rmu_binning) {
+                r_trans = sqrt(r_trans * r_trans + r_par * r_par);
+                r_par /= r_trans;
+            }
+            if (r_par >= p->r_par_max || r_trans >= p->r_trans_max || r_par <= p->r_par_min) {
+                if ((r_par >= p->r_par_max && r_par - p->r_par_max < p->edge_tol) ||
+                    (r_trans >= p->r_trans_max && r_trans - p->r_trans_max < p->edge_tol) ||
+                    (r_par <= p->r_par_min && p->r_par_min - r_par < p->edge_tol))
+                    stats[1] += 1;
+                continue;
+            }
+            double delta_times_weight = delta1[i] * weights1[i] * weights2[j];
+            double weights12 = weights1[i] * weights2[j];
+            double t_par = (r_par - p->r_par_min) / (p->r_par_max - p->r_par_min) * p->num_bins_r_par;
+            double t_trans = r_trans / p->r_trans_max * p->num_bins_r_trans;
+            double bins_r_par = floor(t_par);
+            double bins_r_trans = floor(t_trans);
+            int64_t bins = (int64_t)(bins_r_trans + p->num_bins_r_trans * bins_r_par);
+            if ((r_par != p->r_par_min && edge_distance(t_par, width_p) < p->edge_tol) ||
+                (r_trans != 0 && edge_distance(t_trans, width_t) < p->edge_tol))
+                stats[1] += 1;
+            if (bins < 0 || bins >= nbins) {
+                stats[2] += 1;
+                continue;
+            }
+            rebin_xi[bins] += delta_times_weight;
+            rebin_weight[bins] += weights12;
+            rebin_r_par[bins] += r_par * weights12;
+            rebin_r_trans[bins] += r_trans * weights12;
+            rebin_z[bins] += z * weights12;
+            rebin_num_pairs[bins] += 1;
+            stats[0] += 1;
+        }
+    }
+}
+
+/*
  * Many healpix rows at once.  Pairs are grouped by row (pair_row non-decreasing is NOT
  * required: the pairs of each row are visited in their input order).  Forest k owns
  * pixels [offsets[k], offsets[k+1]) of the pixel arrays.  Outputs are [num_rows * nb].

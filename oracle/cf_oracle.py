@@ -24,6 +24,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBRARY = os.path.join(HERE, "liboracle_cf.so")
 SMALL_ANGLE_CUT_OFF = 2.0 / 3600.0 * np.pi / 180.0
+SPEED_LIGHT_KMS = 299792458.0 / 1000.0
 
 
 def build():
@@ -134,6 +135,10 @@ def lib():
         _lib.oracle_dmat_rows.argtypes = ([ctypes.POINTER(OracleParams)] + [P] * 8
                                           + [ctypes.c_int64] + [P] * 4 + [P] * 7)
         _lib.oracle_dmat_rows.restype = ctypes.c_int
+        _lib.oracle_xcf_pair.argtypes = ([ctypes.POINTER(OracleParams)] + [P] * 5
+                                         + [ctypes.c_int64] + [P] * 5 + [ctypes.c_int64]
+                                         + [P] * 7)
+        _lib.oracle_xcf_pair.restype = None
     return _lib
 
 
@@ -335,3 +340,87 @@ def compute_dmat(params, data, healpixs, neighbours, reject):
                           d1.order, d2.order, stats)
     return (weights_dmat, dmat.reshape(nb, nm), eff[0], eff[1], eff[2], eff[3], num_pairs,
             int(num_pairs_used)), stats
+
+
+# ---------------------------------------------------------------------------
+# forest x quasar cross-correlation (py/picca/xcf.py)
+# ---------------------------------------------------------------------------
+
+def xcf_pair(params, z1, r1, d1, w1, delta1, z2, r2, d2, w2, ang, out, stats):
+    """One forest against its quasar neighbours; ``out`` updated in place."""
+    a = [_f(x) for x in (z1, r1, d1, w1, delta1)]
+    b = [_f(x) for x in (z2, r2, d2, w2, ang)]
+    lib().oracle_xcf_pair(ctypes.byref(params), *[_p(x) for x in a], len(a[0]),
+                          *[_p(x) for x in b[:4]], _p(b[4]), len(b[0]),
+                          *[_p(x) for x in out], _p(stats))
+
+
+def xcf_neighbours(data, objs, healpixs, ang_max, r_par_min, r_par_max, r_trans_max,
+                   rmu_binning=False, ang_correlation=False, zerr_cut_deg=None,
+                   zerr_cut_kms=None):
+    """fill_neighs of the cross-correlation (py/picca/xcf.py:71-128) by brute force: every
+    quasar of every healpix (ascending, catalogue order), minus the same thingid, the
+    angle cut, the quasar-pair zerr cut and the r_comov range cut.  Returns
+    {id(delta): (objects, angles)}."""
+    obj_list = [o for hp in sorted(objs) for o in objs[hp]]
+    x2 = np.array([o.x_cart for o in obj_list])
+    y2 = np.array([o.y_cart for o in obj_list])
+    z2 = np.array([o.z_cart for o in obj_list])
+    ra2 = np.array([o.ra for o in obj_list])
+    dec2 = np.array([o.dec for o in obj_list])
+    thing = np.array([o.thingid for o in obj_list])
+    zq = np.array([o.z_qso for o in obj_list])
+    rc = np.array([o.r_comov if o.r_comov is not None else np.nan for o in obj_list])
+    out = {}
+    for hp in healpixs:
+        for d in data[hp]:
+            keep = thing != d.thingid
+            cos = x2 * d.x_cart + y2 * d.y_cart + z2 * d.z_cart
+            cos = np.clip(cos, -1.0, 1.0)
+            ang = np.arccos(cos)
+            close = ((np.absolute(ra2 - d.ra) < SMALL_ANGLE_CUT_OFF)
+                     & (np.absolute(dec2 - d.dec) < SMALL_ANGLE_CUT_OFF))
+            ang[close] = np.sqrt((dec2[close] - d.dec) ** 2
+                                 + (d.cos_dec * (ra2[close] - d.ra)) ** 2)
+            w = keep & (ang < ang_max)
+            if zerr_cut_deg is not None:
+                ang_deg = 180.0 / np.pi * ang
+                z_qq = 0.5 * (d.z_qso + zq)
+                dv_kms = np.abs(d.z_qso - zq) / (1 + z_qq) * SPEED_LIGHT_KMS
+                w &= ~((ang_deg < zerr_cut_deg) & (dv_kms < zerr_cut_kms))
+            if not ang_correlation:
+                f = r_trans_max if rmu_binning else 1
+                w &= (d.r_comov[0] - rc) * np.cos(ang / 2.0) < r_par_max * f
+                w &= (d.r_comov[-1] - rc) * np.cos(ang / 2.0) > r_par_min * f
+            idx = np.where(w)[0]
+            out[id(d)] = ([obj_list[k] for k in idx], ang[idx])
+    return out
+
+
+def compute_xcf(params, data, healpixs, neighbours):
+    """compute_xi of the cross-correlation (py/picca/xcf.py:131-220) over ``healpixs``:
+    normalised (we, xi, rp, rt, z, nb) and stats."""
+    nb = params.num_bins_r_par * params.num_bins_r_trans
+    out = [np.zeros(nb) for _ in range(5)] + [np.zeros(nb, dtype=np.int64)]
+    stats = np.zeros(3, dtype=np.int64)
+    for hp in healpixs:
+        for d in data[hp]:
+            objs_n, ang = neighbours[id(d)]
+            if len(objs_n) == 0:
+                continue
+            zq = np.array([o.z_qso for o in objs_n])
+            wq = np.array([o.weights for o in objs_n])
+            if params.ang_correlation:
+                lam = np.array([10.0 ** o.log_lambda for o in objs_n])
+                xcf_pair(params, d.z, 10.0 ** d.log_lambda, 10.0 ** d.log_lambda, d.weights,
+                         d.delta, zq, lam, lam, wq, ang, out, stats)
+            else:
+                rq = np.array([o.r_comov for o in objs_n])
+                dq = np.array([o.dist_m for o in objs_n])
+                xcf_pair(params, d.z, d.r_comov, d.dist_m, d.weights, d.delta, zq, rq, dq, wq,
+                         ang, out, stats)
+    we, xi, rp, rt, z, npairs = out
+    w = we > 0
+    for arr in (xi, rp, rt, z):
+        arr[w] /= we[w]
+    return (we, xi, rp, rt, z, npairs), stats

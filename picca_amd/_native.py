@@ -91,6 +91,8 @@ _PROTOTYPES = {
     "picca_xi_download": [_P, _P, _P, _P, _P, _P, _P],
     "picca_dmat_run": [_P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "picca_diag_copy": [_P, _I64, _P],
+    "picca_xcf_set_pairs": [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _I32, _I32],
+    "picca_xcf_execute": [_P, _P, _P],
     "picca_cont_set_forests": [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "picca_cont_fit": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "picca_mlp_init": [_P, _I32, _I32, _P, _P, _P, _P],

@@ -429,6 +429,138 @@ int picca_xi_download(picca_ctx *ctx, double *weights, double *xi, double *r_par
     return PICCA_OK;
 }
 
+int picca_xcf_set_pairs(picca_ctx *ctx, int64_t num_pairs, const int32_t *pair_forest,
+                        const double *obj_r, const double *obj_d, const double *obj_z,
+                        const double *obj_w, const double *pair_ang, const int32_t *pair_row,
+                        int32_t num_rows, int32_t angular) {
+    if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
+    if (num_pairs < 0 || num_rows < 0) return fail(PICCA_E_VALUE, "negative size");
+    if (num_pairs > 0 && (!pair_forest || !obj_r || !obj_d || !obj_z || !obj_w || !pair_ang ||
+                          !pair_row))
+        return fail(PICCA_E_VALUE, "pair arrays must not be NULL");
+    const int64_t nf = ctx->num_forests;
+    // groups: maximal runs of pairs of one forest; rows contiguous
+    std::vector<int64_t> begin;
+    std::vector<int32_t> forest, row;
+    std::vector<uint8_t> row_seen((size_t)std::max(num_rows, 1), 0);
+    for (int64_t k = 0; k < num_pairs; ++k) {
+        if (pair_forest[k] < 0 || pair_forest[k] >= nf)
+            return fail(PICCA_E_VALUE, "pair forest out of range of the forest store");
+        if (pair_row[k] < 0 || pair_row[k] >= num_rows)
+            return fail(PICCA_E_VALUE, "pair row out of range");
+        if (!std::isfinite(obj_r[k]) || !std::isfinite(obj_d[k]) || !std::isfinite(obj_z[k]) ||
+            !std::isfinite(obj_w[k]) || !(pair_ang[k] >= 0.0) || !(pair_ang[k] <= M_PI))
+            return fail(PICCA_E_VALUE, "quasar values must be finite, angles in [0, pi]");
+        const bool new_group = k == 0 || pair_forest[k] != pair_forest[k - 1] ||
+                               pair_row[k] != pair_row[k - 1];
+        if (new_group) {
+            if (k > 0 && pair_row[k] != pair_row[k - 1]) {
+                if (row_seen[pair_row[k]])
+                    return fail(PICCA_E_VALUE, "pairs of one row must be contiguous");
+            }
+            row_seen[pair_row[k]] = 1;
+            begin.push_back(k);
+            forest.push_back(pair_forest[k]);
+            row.push_back(pair_row[k]);
+        }
+    }
+    begin.push_back(num_pairs);
+    // records; within a group sorted by the quasar's r (the kernel's pruning walks windows
+    // of neighbouring r), which only reorders the additions of the sums
+    std::vector<picca::XcfPairRecord> rec((size_t)std::max<int64_t>(num_pairs, 1));
+    const int64_t ngroups = (int64_t)forest.size();
+    parallel_for(ngroups, [&](int64_t lo, int64_t hi) {
+        std::vector<int64_t> idx;
+        for (int64_t g = lo; g < hi; ++g) {
+            const int64_t b = begin[g], e = begin[g + 1];
+            idx.resize((size_t)(e - b));
+            for (int64_t k = b; k < e; ++k) idx[k - b] = k;
+            if (!angular)
+                std::stable_sort(idx.begin(), idx.end(),
+                                 [&](int64_t x, int64_t y) { return obj_r[x] < obj_r[y]; });
+            for (int64_t k = b; k < e; ++k) {
+                const int64_t s = idx[k - b];
+                picca::XcfPairRecord &r = rec[k];
+                r.r = obj_r[s];
+                r.d = obj_d[s];
+                r.z = obj_z[s];
+                r.w = obj_w[s];
+                if (angular) {
+                    r.c = pair_ang[s];
+                    r.s = 0.0;
+                } else {
+                    r.c = std::cos(pair_ang[s] / 2);
+                    r.s = std::sin(pair_ang[s] / 2);
+                }
+            }
+        }
+    });
+    PICCA_HIP_CHECK(hipSetDevice(ctx->device));
+    PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->have_result = false;
+    PICCA_HIP_CHECK(ctx->x_rec.upload(rec.data(), (size_t)num_pairs, ctx->stream));
+    PICCA_HIP_CHECK(ctx->x_forest.upload(forest.data(), forest.size(), ctx->stream));
+    PICCA_HIP_CHECK(ctx->x_row.upload(row.data(), row.size(), ctx->stream));
+    PICCA_HIP_CHECK(ctx->x_begin.upload(begin.data(), begin.size(), ctx->stream));
+    PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->xcf_pairs = num_pairs;
+    ctx->xcf_rows = num_rows;
+    ctx->xcf_groups = ngroups;
+    return PICCA_OK;
+}
+
+int picca_xcf_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_report *report) {
+    if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
+    picca::XiConst C;
+    int mode = 0;
+    int status = build_xi_const(params, C, mode);
+    if (status != PICCA_OK) return status;
+    const size_t static_lds = 1024;
+    const size_t hist = (size_t)C.nb * picca::XI_HIST_BIN_BYTES;
+    const bool lds_hist = (int64_t)(hist + static_lds) <= ctx->lds_limit;
+    const size_t lds = lds_hist ? hist : 16;
+    PICCA_HIP_CHECK(hipSetDevice(ctx->device));
+    const int64_t nrows = ctx->xcf_rows;
+    const int64_t nout = nrows * C.nb;
+    PICCA_HIP_CHECK(ctx->o_bins.reserve((size_t)std::max<int64_t>(nout, 1)));
+    PICCA_HIP_CHECK(ctx->queue.reserve(1));
+    PICCA_HIP_CHECK(ctx->stats.reserve(picca::XI_NSTATS));
+    PICCA_HIP_CHECK(picca::launch_zero_u64(reinterpret_cast<unsigned long long *>(ctx->o_bins.ptr),
+                                           nout * (int64_t)(sizeof(picca::XiBin) / 8), ctx->stream));
+    PICCA_HIP_CHECK(picca::launch_zero_u64(ctx->queue.ptr, 1, ctx->stream));
+    PICCA_HIP_CHECK(picca::launch_zero_u64(ctx->stats.ptr, picca::XI_NSTATS, ctx->stream));
+    picca::ForestView F{ctx->pix.ptr, ctx->z.ptr, ctx->r.ptr, ctx->d.ptr, ctx->w.ptr, ctx->dw.ptr,
+                        ctx->ll.ptr, ctx->off.ptr, ctx->zq.ptr, ctx->order.ptr, ctx->mono.ptr};
+    picca::XcfView X{ctx->x_rec.ptr, ctx->x_forest.ptr, ctx->x_begin.ptr, ctx->x_row.ptr,
+                     ctx->xcf_groups};
+    picca::XiOut O{ctx->o_bins.ptr, ctx->queue.ptr, ctx->stats.ptr};
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->num_cus, ctx->xcf_groups));
+    PICCA_HIP_CHECK(hipEventRecord(ctx->ev_start, ctx->stream));
+    if (ctx->xcf_groups > 0)
+        PICCA_HIP_CHECK(picca::launch_xcf(mode, lds_hist, C, F, X, O, grid, lds, ctx->stream));
+    PICCA_HIP_CHECK(hipEventRecord(ctx->ev_stop, ctx->stream));
+    unsigned long long stats[picca::XI_NSTATS];
+    PICCA_HIP_CHECK(hipMemcpyAsync(stats, ctx->stats.ptr, sizeof(stats), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+    PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    PICCA_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    ctx->out_nb = C.nb;
+    ctx->out_rows = (int32_t)nrows;
+    ctx->have_result = true;
+    if (report) {
+        report->accepted_pairs = (int64_t)stats[picca::XI_STAT_ACCEPTED];
+        report->edge_pairs = (int64_t)stats[picca::XI_STAT_EDGE];
+        report->overflow_pairs = (int64_t)stats[picca::XI_STAT_OVERFLOW];
+        report->visited_pairs = (int64_t)stats[picca::XI_STAT_VISITED];
+        report->wave_steps = (int64_t)stats[picca::XI_STAT_STEPS];
+        report->tasks = (int64_t)stats[picca::XI_STAT_TASKS];
+        report->empty_tasks = (int64_t)stats[picca::XI_STAT_EMPTY_TASKS];
+        report->kernel_ms = (double)ms;
+    }
+    return PICCA_OK;
+}
+
 int picca_diag_copy(picca_ctx *ctx, int64_t bytes, double *kernel_ms) {
     if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
     if (bytes <= 0 || bytes % 16 != 0)

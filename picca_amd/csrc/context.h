@@ -164,6 +164,14 @@ struct picca_ctx {
     DeviceBuffer<int32_t> m_rows;
     int64_t mlp_rows = 0;   // rows of the resident training data
 
+    // forest x quasar pairs (cross-correlation)
+    int64_t xcf_pairs = 0;
+    int32_t xcf_rows = 0;
+    DeviceBuffer<picca::XcfPairRecord> x_rec;
+    DeviceBuffer<int32_t> x_forest, x_row;
+    DeviceBuffer<int64_t> x_begin;
+    int64_t xcf_groups = 0;
+
     // xi results
     int64_t out_nb = 0;
     int32_t out_rows = 0;

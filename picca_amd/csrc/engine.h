@@ -273,6 +273,29 @@ hipError_t launch_mlp_adam(const MlpView &M, const MlpGrads &G, const MlpAdam &A
                            hipStream_t stream);
 hipError_t launch_mlp_pack(const MlpView &M, hipStream_t stream);
 
+// ---------------------------------------------------------------------------------------
+// forest x quasar cross-correlation
+// ---------------------------------------------------------------------------------------
+
+// One (forest, quasar) pair: the quasar's r_comov, dist_m, z, weight and the pair's
+// cos(ang/2), sin(ang/2).  In the angular mode r is the quasar's wavelength and c the
+// angle.
+struct XcfPairRecord {
+    double r, d, z, w, c, s;
+};
+
+struct XcfView {
+    const XcfPairRecord *__restrict__ rec;  // [num_pairs], grouped by forest
+    const int32_t *__restrict__ forest;     // [count] forest of each group
+    const int64_t *__restrict__ begin;      // [count + 1] pair range of each group
+    const int32_t *__restrict__ row;        // [count] output row of each group
+    int64_t count;
+};
+
+hipError_t launch_xcf(int mode, bool lds_hist, const XiConst &C, const ForestView &F,
+                      const XcfView &X, const XiOut &O, int grid, size_t lds,
+                      hipStream_t stream);
+
 hipError_t launch_xi(int mode, bool xcorr, bool general, bool lds_hist, const XiConst &C,
                      const ForestView &F, const PairView &P, const GroupView &G,
                      const XiOut &O, int grid, size_t lds, hipStream_t stream);

@@ -138,6 +138,23 @@ class Engine:
         self.num_pairs = len(a)
         self._pairs_key = key
 
+    def xcf_set_pairs(self, pair_forest, obj_r, obj_d, obj_z, obj_w, pair_ang, pair_row,
+                      num_rows, angular=False):
+        f = as_i32(pair_forest)
+        arrays = [as_f64(a) for a in (obj_r, obj_d, obj_z, obj_w, pair_ang)]
+        row = as_i32(pair_row)
+        check(self._lib.picca_xcf_set_pairs(self._ctx, len(f), ptr(f), *(ptr(a) for a in arrays),
+                                            ptr(row), int(num_rows), int(bool(angular))),
+              "picca_xcf_set_pairs")
+        self.num_rows = int(num_rows)
+
+    def xcf(self, params):
+        report = _native.XiReport()
+        check(self._lib.picca_xcf_execute(self._ctx, ctypes.byref(params), ctypes.byref(report)),
+              "picca_xcf_execute")
+        (we, xi, rp, rt, z), nb = self.xi_download(params)
+        return XiResult(we, xi, rp, rt, z, nb, report.as_dict())
+
     def xi_execute(self, params):
         report = _native.XiReport()
         check(self._lib.picca_xi_execute(self._ctx, ctypes.byref(params), ctypes.byref(report)),

@@ -145,3 +145,96 @@ def read_deltas(in_dir, nside, lambda_abs, alpha, z_ref, cosmo, max_num_spec=Non
             delta.project()
         data.setdefault(int(healpix), []).append(delta)
     return data, num_data, z_min, z_max
+
+
+def read_drq(drq_filename, z_min=0, z_max=10.0, keep_bal=False, bi_max=None, mode="sdss"):
+    """Quasars of a DRQ catalogue (``read_drq``, py/picca/io.py:160-294): the selection
+    id > 0, ra != dec, ra != 0, dec != 0, z_min <= z < z_max, the BAL cuts, and RA/DEC
+    converted to radians.  Returns a dict of column arrays."""
+    userprint("Reading catalog from ", drq_filename)
+    hdu = FITS(drq_filename)[1]
+    names = hdu.get_colnames()
+    catalog = {name: np.asarray(hdu[name][:]) for name in names}
+    if "desi" in mode and "TARGETID" in catalog:
+        obj_id_name = "TARGETID"
+        if "TARGET_RA" in catalog:
+            catalog["RA"] = catalog.pop("TARGET_RA")
+            catalog["DEC"] = catalog.pop("TARGET_DEC")
+    else:
+        obj_id_name = "THING_ID"
+    if "Z" not in catalog:
+        if "Z_VI" in catalog:
+            catalog["Z"] = catalog.pop("Z_VI")
+            userprint("Z not found (new DRQ >= DRQ14 style), using Z_VI (DRQ <= DRQ12)")
+        else:
+            userprint("ERROR: No valid column for redshift found in ", drq_filename)
+            return None
+    if mode == "desi_mocks":
+        for key in ("RA", "DEC"):
+            catalog[key] = catalog[key].astype("float64")
+    w = np.ones(len(catalog["Z"]), dtype=bool)
+    w &= catalog[obj_id_name] > 0
+    w &= catalog["RA"] != catalog["DEC"]
+    w &= catalog["RA"] != 0.0
+    w &= catalog["DEC"] != 0.0
+    w &= catalog["Z"] >= z_min
+    w &= catalog["Z"] < z_max
+    if not keep_bal and bi_max is None:
+        if "BAL_FLAG_VI" in catalog:
+            w &= catalog["BAL_FLAG_VI"] == 0
+        else:
+            userprint("WARNING: BAL_FLAG_VI not found")
+    if bi_max is not None:
+        if "BI_CIV" not in catalog:
+            raise ValueError("--bi-max set but no BI_CIV field in HDU")
+        w &= catalog["BI_CIV"] <= bi_max
+    index = np.where(w)[0]
+    out = {name: values[index] for name, values in catalog.items()}
+    out["RA"] = np.radians(out["RA"])
+    out["DEC"] = np.radians(out["DEC"])
+    out["_id_name"] = obj_id_name
+    return out
+
+
+def read_objects(filename, nside, z_min, z_max, alpha, z_ref, cosmo, mode="sdss",
+                 keep_bal=True):
+    """Quasar objects by healpix (``read_objects``, py/picca/io.py:515-612): QSO
+    records with weights ((1+z)/(1+z_ref))^(alpha-1) and, with a cosmology, r_comov and
+    D_M at z_qso.  Returns (objs, z_min of the catalogue)."""
+    from .data import QSO
+    catalog = read_drq(filename, z_min=z_min, z_max=z_max, keep_bal=keep_bal, mode=mode)
+    phi = catalog["RA"]
+    theta = np.pi / 2.0 - catalog["DEC"]
+    healpixs = ang2pix(nside, theta, phi)
+    if healpixs.size == 0:
+        raise AssertionError()
+    userprint("Reading objects ")
+    objs = {}
+    for healpix in np.unique(healpixs):
+        index = np.where(healpixs == healpix)[0]
+        if "TARGETID" in catalog:
+            fiber = catalog["FIBER"] if "FIBER" in catalog else catalog["TARGETID"]
+            night = None
+            for col in ("LAST_NIGHT", "NIGHT"):
+                if col in catalog:
+                    night = catalog[col]
+                    break
+            if night is None or mode == "desi_mocks":
+                night = catalog["TARGETID"]
+            objs[int(healpix)] = [
+                QSO(int(catalog["TARGETID"][k]), float(catalog["RA"][k]),
+                    float(catalog["DEC"][k]), float(catalog["Z"][k]),
+                    int(catalog["TARGETID"][k]), int(night[k]), int(fiber[k]))
+                for k in index]
+        else:
+            objs[int(healpix)] = [
+                QSO(int(catalog["THING_ID"][k]), float(catalog["RA"][k]),
+                    float(catalog["DEC"][k]), float(catalog["Z"][k]), int(catalog["PLATE"][k]),
+                    int(catalog["MJD"][k]), int(catalog["FIBERID"][k]))
+                for k in index]
+        for obj in objs[int(healpix)]:
+            obj.weights = ((1.0 + obj.z_qso) / (1.0 + z_ref)) ** (alpha - 1.0)
+            if cosmo is not None:
+                obj.r_comov = float(cosmo.get_r_comov(obj.z_qso))
+                obj.dist_m = float(cosmo.get_dist_m(obj.z_qso))
+    return objs, catalog["Z"].min()

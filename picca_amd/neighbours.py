@@ -51,16 +51,18 @@ def _catalog(store, index):
     }
 
 
-def find_neighbours(store, source_index, target_index, ang_max, auto):
+def find_neighbours(store, source_index, target_index, ang_max, auto, target_store=None):
     """Neighbour lists of ``source_index`` among ``target_index`` (both into ``store``).
 
     ``target_index`` must already be in the reference's neighbour order (healpix
     ascending, then file order).  Returns a NeighbourList whose ``target`` entries are
-    positions in ``target_index``.
+    positions in ``target_index``.  ``target_store`` (any object with the same position
+    and id arrays, e.g. a quasar catalogue) holds the targets when they are not forests of
+    ``store``.
     """
     lib = _native.load()
     src = _catalog(store, source_index)
-    tgt = _catalog(store, target_index)
+    tgt = _catalog(store if target_store is None else target_store, target_index)
     n_src = len(src["x"])
     n_tgt = len(tgt["x"])
     ang_max = float(ang_max)

@@ -1,0 +1,258 @@
+"""Forest x quasar cross-correlation on the MI355X engine: mirror of ``picca.xcf``.
+
+Module globals with picca's names (py/picca/xcf.py:20-67), and:
+
+* ``fill_neighs`` (:71-128): for every delta, the quasars of the healpix pixels near it
+  (healpix ascending, catalogue order) minus the same thingid, minus angle >= ang_max,
+  minus the quasar pairs cut by the redshift-error cut, minus (outside the angular mode)
+  those whose r_comov puts every pixel out of the r_par range;
+* ``compute_xi`` (:131-220): the pair kernel over every (delta, neighbour) of the given
+  healpix pixels in one GPU launch, normalised like the reference;
+* ``compute_xi_rows``: one output row per healpix in one launch (what picca_xcf needs);
+* ``compute_xi_forest_pairs_fast``: the in-place per-forest kernel (:223-322), as a
+  launch of one forest.
+
+The neighbour candidates come from the engine's grid search; the angles are
+``QSO.get_angle_between`` evaluated with numpy exactly as the reference does.
+"""
+import sys
+
+import numpy as np
+
+from . import engine as _engine
+from .constants import SPEED_LIGHT
+from .neighbours import find_neighbours
+from .store import ForestStore
+
+num_bins_r_par = None
+num_bins_r_trans = None
+num_model_bins_r_trans = None
+num_model_bins_r_par = None
+r_par_max = None
+r_par_min = None
+r_trans_max = None
+z_cut_max = None
+z_cut_min = None
+z_min_pairs = None
+z_max_pairs = None
+ang_max = None
+nside = None
+zerr_cut_deg = None
+zerr_cut_kms = None
+counter = None
+num_data = None
+z_ref = None
+alpha = None
+alpha_obj = None
+lambda_abs = None
+alpha_abs = None
+data = None
+objs = None
+reject = None
+lock = None
+x_correlation = True
+ang_correlation = False
+rmu_binning = False
+redshift_evolution_in_distortion_matrix = True
+
+device = 0
+last_report = None
+
+
+class _Objects:
+    """The quasar catalogue as arrays, in the reference's neighbour order."""
+
+    def __init__(self, objs_dict):
+        self.healpixs = sorted(objs_dict)
+        self.objects = [o for hp in self.healpixs for o in objs_dict[hp]]
+
+        def arr(name, default=np.nan):
+            return np.array([getattr(o, name, default) if getattr(o, name, None) is not None
+                             else default for o in self.objects], dtype=np.float64)
+        self.x_cart = arr("x_cart")
+        self.y_cart = arr("y_cart")
+        self.z_cart = arr("z_cart")
+        self.ra = arr("ra")
+        self.dec = arr("dec")
+        self.cos_dec = arr("cos_dec")
+        self.thingid = np.array([int(o.thingid) for o in self.objects], dtype=np.int64)
+        self.z_qso = arr("z_qso")
+        self.r_comov = arr("r_comov")
+        self.dist_m = arr("dist_m")
+        self.weights = arr("weights")
+        self.log_lambda = arr("log_lambda")
+        self.key = (id(objs_dict), len(self.objects))
+
+
+class _Catalogue:
+    def __init__(self, data_dict, objs_dict, angular):
+        self.healpixs = sorted(data_dict)
+        self.deltas = [d for hp in self.healpixs for d in data_dict[hp]]
+        self.store = ForestStore.from_deltas(self.deltas, angular=angular)
+        self.index = {id(d): k for k, d in enumerate(self.deltas)}
+        self.objects = _Objects(objs_dict)
+        self.key = (id(data_dict), id(objs_dict), bool(angular), len(self.deltas))
+
+    def index_of(self, delta):
+        try:
+            return self.index[id(delta)]
+        except KeyError:
+            raise RuntimeError("delta is not part of xcf.data") from None
+
+
+_catalogue_cache = None
+
+
+def _catalogue():
+    global _catalogue_cache
+    if data is None or objs is None:
+        raise RuntimeError("xcf.data and xcf.objs must be set")
+    key = (id(data), id(objs), bool(ang_correlation))
+    if _catalogue_cache is None or _catalogue_cache.key[:3] != key:
+        _catalogue_cache = _Catalogue(data, objs, bool(ang_correlation))
+    return _catalogue_cache
+
+
+def current_params():
+    return _engine.params_from_globals(sys.modules[__name__])
+
+
+def fill_neighs(healpixs):
+    """Store the quasar neighbours of every delta of ``healpixs`` in ``delta.neighbours``
+    (an object array) and their angles in ``delta.neighbour_angles``."""
+    cat = _catalogue()
+    sources = [d for hp in healpixs for d in data[hp]]
+    if not sources:
+        return
+    src_index = np.array([cat.index_of(d) for d in sources], dtype=np.int64)
+    obj = cat.objects
+    nl = find_neighbours(cat.store, src_index, np.arange(len(obj.objects), dtype=np.int64),
+                         ang_max, auto=False, target_store=obj)
+    for s, delta in enumerate(sources):
+        lo, hi = nl.offsets[s], nl.offsets[s + 1]
+        tgt = nl.target[lo:hi]
+        ang = nl.angle[lo:hi]
+        w = np.ones(len(tgt), dtype=bool)
+        if zerr_cut_deg is not None:
+            ang_deg = 180.0 / np.pi * ang
+            obj_z_qso = obj.z_qso[tgt]
+            z_qq = 0.5 * (delta.z_qso + obj_z_qso)
+            dv_kms = np.abs(delta.z_qso - obj_z_qso) / (1 + z_qq)
+            dv_kms *= SPEED_LIGHT
+            zerr_cut_mask = ang_deg < zerr_cut_deg
+            zerr_cut_mask &= dv_kms < zerr_cut_kms
+            w &= ~zerr_cut_mask
+        if not ang_correlation:
+            r_comov = obj.r_comov[tgt]
+            f = r_trans_max if rmu_binning else 1
+            w &= (delta.r_comov[0] - r_comov) * np.cos(ang / 2.0) < r_par_max * f
+            w &= (delta.r_comov[-1] - r_comov) * np.cos(ang / 2.0) > r_par_min * f
+        delta.neighbours = np.array([obj.objects[t] for t in tgt[w]], dtype=object)
+        delta.neighbour_index = tgt[w]
+        delta.neighbour_angles = ang[w]
+
+
+def _pairs(cat, deltas_by_row):
+    """Flat pair arrays for a list of (row, deltas) in row order."""
+    obj = cat.objects
+    forest, oidx, ang, row = [], [], [], []
+    for r, deltas in deltas_by_row:
+        for delta in deltas:
+            idx = getattr(delta, "neighbour_index", None)
+            if idx is None or len(idx) == 0:
+                continue
+            forest.append(np.full(len(idx), cat.index_of(delta), dtype=np.int32))
+            oidx.append(np.asarray(idx, dtype=np.int64))
+            ang.append(np.asarray(delta.neighbour_angles, dtype=np.float64))
+            row.append(np.full(len(idx), r, dtype=np.int32))
+    if not forest:
+        return None
+    forest = np.concatenate(forest)
+    oidx = np.concatenate(oidx)
+    ang = np.concatenate(ang)
+    row = np.concatenate(row)
+    if ang_correlation:
+        r2 = 10.0 ** obj.log_lambda[oidx]
+        d2 = r2
+    else:
+        r2 = obj.r_comov[oidx]
+        d2 = obj.dist_m[oidx]
+    return forest, r2, d2, obj.z_qso[oidx], obj.weights[oidx], ang, row
+
+
+def _run(cat, pairs, num_rows):
+    global last_report
+    params = current_params()
+    eng = _engine.get_engine(device)
+    eng.set_forests(cat.store, key=cat.key)
+    if pairs is None:
+        nb = params.num_bins_r_par * params.num_bins_r_trans
+        zeros = [np.zeros((num_rows, nb)) for _ in range(5)]
+        last_report = None
+        return zeros + [np.zeros((num_rows, nb), dtype=np.int64)]
+    forest, r2, d2, z2, w2, ang, row = pairs
+    eng.xcf_set_pairs(forest, r2, d2, z2, w2, ang, row, num_rows, angular=bool(ang_correlation))
+    res = eng.xcf(params)
+    last_report = res.report
+    return [res.weights, res.xi, res.r_par, res.r_trans, res.z, res.num_pairs]
+
+
+def _normalise(we, xi, rp, rt, z):
+    w = we > 0
+    for arr in (xi, rp, rt, z):
+        arr[w] /= we[w]
+
+
+def compute_xi(healpixs):
+    """compute_xi (py/picca/xcf.py:131-220) over the deltas of ``healpixs``; consumes
+    their neighbour lists."""
+    cat = _catalogue()
+    deltas = [d for hp in healpixs for d in data[hp]]
+    out = _run(cat, _pairs(cat, [(0, deltas)]), 1)
+    we, xi, rp, rt, z = (a[0].copy() for a in out[:5])
+    num_pairs = out[5][0].copy()
+    _normalise(we, xi, rp, rt, z)
+    for d in deltas:
+        d.neighbours = None
+    return we, xi, rp, rt, z, num_pairs
+
+
+def compute_xi_rows(healpixs):
+    """One normalised compute_xi row per healpix, all in one launch (fills the neighbours
+    first)."""
+    fill_neighs(healpixs)
+    cat = _catalogue()
+    groups = [(r, data[hp]) for r, hp in enumerate(healpixs)]
+    out = _run(cat, _pairs(cat, groups), len(healpixs))
+    rows = []
+    for r in range(len(healpixs)):
+        we, xi, rp, rt, z = (a[r].copy() for a in out[:5])
+        _normalise(we, xi, rp, rt, z)
+        rows.append((we, xi, rp, rt, z, out[5][r].copy()))
+    for hp in healpixs:
+        for d in data[hp]:
+            d.neighbours = None
+    return rows
+
+
+def compute_xi_forest_pairs_fast(z1, r_comov1, dist_m1, weights1, delta1, z2, r_comov2,
+                                 dist_m2, weights2, ang, rebin_weight, rebin_xi, rebin_r_par,
+                                 rebin_r_trans, rebin_z, rebin_num_pairs):
+    """Adds one forest against its quasars to the caller's arrays (py/picca/xcf.py:223-322);
+    runs as a GPU launch of one forest."""
+    n1 = len(z1)
+    store = ForestStore(np.array([0, n1], dtype=np.int64), z1, r_comov1, dist_m1, weights1,
+                        delta1, np.zeros(1), np.zeros(1), np.zeros(1), np.array([0]))
+    params = current_params()
+    eng = _engine.get_engine(device)
+    eng.set_forests(store)
+    n2 = len(z2)
+    eng.xcf_set_pairs(np.zeros(n2, np.int32), r_comov2, dist_m2, z2, weights2, ang,
+                      np.zeros(n2, np.int32), 1, angular=bool(ang_correlation))
+    res = eng.xcf(params)
+    rebin_weight += res.weights[0]
+    rebin_xi += res.xi[0]
+    rebin_r_par += res.r_par[0]
+    rebin_r_trans += res.r_trans[0]
+    rebin_z += res.z[0]
+    rebin_num_pairs += res.num_pairs[0]

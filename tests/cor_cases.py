@@ -143,6 +143,84 @@ class AngularSetup:
         self.data2 = None
 
 
+CAT = os.path.join(GOLDEN, "cat.fits")
+XCF_COMMON = ["--in-dir", DELTA_LYA, "--drq", CAT, "--rp-min", "-60.0", "--rp-max", "+60.0",
+              "--rt-max", "+60.0", "--np", "30", "--nt", "15", "--nproc", "1",
+              "--z-evol-obj", "1.", "--in-attributes", ATTRIBUTES]
+XCF_CASES = {
+    "xcf": list(XCF_COMMON),
+    "xcf_zcuts": XCF_COMMON + ["--z-min-pairs", "2.25", "--z-max-pairs", "2.3",
+                               "--z-min-sources", "2.3", "--z-max-sources", "2.5"],
+}
+
+
+def _remove_mean_lambda_obs(data, z_min, z_max, lambda_abs):
+    """py/picca/bin/picca_xcf.py:436-468 with py/picca/prep_del.py:13-60, restated."""
+    step = None
+    for hp in data:
+        for d in data[hp]:
+            s = np.asarray([d.log_lambda[k] - d.log_lambda[k - 1]
+                            for k in range(1, d.log_lambda.size)]).min()
+            step = s if step is None else min(s, step)
+    ll_min = np.log10((z_min + 1.0) * lambda_abs) - step / 2.0
+    ll_max = np.log10((z_max + 1.0) * lambda_abs) + step / 2.0
+    nbins = int((ll_max - ll_min) / step) + 1
+    stack = np.zeros(nbins)
+    weight = np.zeros(nbins)
+    for hp in sorted(data):
+        for d in data[hp]:
+            bins = ((d.log_lambda - ll_min) / step + 0.5).astype(int)
+            r = np.bincount(bins, weights=d.delta * d.weights)
+            stack[:len(r)] += r
+            r = np.bincount(bins, weights=d.weights)
+            weight[:len(r)] += r
+    w = weight > 0
+    stack[w] /= weight[w]
+    for hp in data:
+        for d in data[hp]:
+            bins = ((d.log_lambda - ll_min) / step + 0.5).astype(int)
+            d.delta -= stack[bins]
+
+
+class XcfSetup:
+    """Globals, deltas and quasars of one picca_xcf test command, read as picca_xcf
+    does (py/picca/bin/picca_xcf.py:337-490)."""
+
+    def __init__(self, argv):
+        args = {}
+        k = 0
+        while k < len(argv):
+            args[argv[k]] = argv[k + 1]
+            k += 2
+        self.r_par_min = float(args["--rp-min"])
+        self.r_par_max = float(args["--rp-max"])
+        self.r_trans_max = float(args["--rt-max"])
+        self.num_bins_r_par = int(args["--np"])
+        self.num_bins_r_trans = int(args["--nt"])
+        self.num_model_bins_r_par = None
+        self.num_model_bins_r_trans = None
+        self.z_min_pairs = float(args["--z-min-pairs"]) if "--z-min-pairs" in args else None
+        self.z_max_pairs = float(args["--z-max-pairs"]) if "--z-max-pairs" in args else None
+        self.zerr_cut_deg = self.zerr_cut_kms = None
+        self.z_ref = 2.25
+        self.alpha = 2.9
+        self.alpha2 = 1.0
+        self.x_correlation = True
+        self.rmu_binning = False
+        self.ang_correlation = False
+        self.remove_same_half_plate_close_pairs = False
+        self.redshift_evolution_in_distortion_matrix = True
+        cosmo = constants.Cosmo(Om=0.315, verbose=False)
+        self.objs, z_min2 = io.read_objects(CAT, 16, 0, 10, float(args["--z-evol-obj"]), 2.25,
+                                            cosmo)
+        self.data, self.num_data, z_min, z_max = io.read_deltas(
+            DELTA_LYA, 16, constants.ABSORBER_IGM["LYA"], 2.9, 2.25, cosmo,
+            z_min_qso=float(args.get("--z-min-sources", 0.0)),
+            z_max_qso=float(args.get("--z-max-sources", 10.0)), delta_attributes=ATTRIBUTES)
+        _remove_mean_lambda_obs(self.data, z_min, z_max, constants.ABSORBER_IGM["LYA"])
+        self.ang_max = utils.compute_ang_max(cosmo, self.r_trans_max, z_min, z_min2)
+
+
 def reduce_rows(rows):
     """The parent-process reduction of py/picca/bin/picca_cf.py:466-484."""
     rows = np.array(rows)

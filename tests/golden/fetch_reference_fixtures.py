@@ -30,6 +30,12 @@ FILES = {
     "py/picca/tests/data/test_cor/dmat_cross.fits.gz": "cor/dmat_cross.fits.gz",
     "py/picca/tests/data/test_cor/dmat_zcuts.fits.gz": "cor/dmat_zcuts.fits.gz",
     "py/picca/tests/data/test_cor/dmat_cross_zcuts.fits.gz": "cor/dmat_cross_zcuts.fits.gz",
+    # quasar catalogue and cross-correlation goldens
+    "py/picca/tests/data/test_delta/cat.fits": "cat.fits",
+    "py/picca/tests/data/test_cor/xcf.fits.gz": "cor/xcf.fits.gz",
+    "py/picca/tests/data/test_cor/xcf_zcuts.fits.gz": "cor/xcf_zcuts.fits.gz",
+    "py/picca/tests/data/test_cor/xdmat.fits.gz": "cor/xdmat.fits.gz",
+    "py/picca/tests/data/test_cor/xdmat_zcuts.fits.gz": "cor/xdmat_zcuts.fits.gz",
 }
 
 

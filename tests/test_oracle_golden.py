@@ -70,3 +70,24 @@ def test_oracle_reproduces_golden_angular():
     golden = cor_cases.golden_columns("cf_angl")
     cor_cases.compare_to_golden(result, golden)
     assert stats[0] == golden["NB"].sum() > 0
+
+
+@pytest.mark.parametrize("case", ["xcf", "xcf_zcuts"])
+def test_oracle_reproduces_golden_xcf(case):
+    setup = cor_cases.XcfSetup(cor_cases.XCF_CASES[case])
+    params = O.params_from_module(setup)
+    healpixs = sorted(setup.data)
+    neigh = O.xcf_neighbours(setup.data, setup.objs, healpixs, setup.ang_max,
+                             setup.r_par_min, setup.r_par_max, setup.r_trans_max)
+    rows = []
+    stats = np.zeros(3, dtype=np.int64)
+    for hp in healpixs:
+        res, st = O.compute_xcf(params, setup.data, [hp], neigh)
+        rows.append(res)
+        stats += st
+    result = cor_cases.reduce_rows(rows)
+    result["HEALPID"] = np.array(healpixs)
+    golden = cor_cases.golden_columns(case)
+    cor_cases.compare_to_golden(result, golden)
+    assert stats[0] == golden["NB"].sum() > 0
+    assert stats[2] == 0
