@@ -422,24 +422,28 @@ def bench_continuum(args):
     train_s = time.time() - t1
     log("seed network: %d Adam steps in %.1f s, loss %.3e -> %.3e"
         % (len(losses), train_s, losses[:20].mean(), losses[-20:].mean()))
-    plain_ms, seeded_ms, infer_s = [], [], []
+    # both fits stop at Migrad's kind of rule, an absolute EDM bound (here 1e-5, so the
+    # final chi2 is within ~1e-5 of the minimum); the reference minimum comes from the
+    # plain fit run to EDM < 1e-12 chi2
+    exact = continuum.fit_arrays(test_in, order=1)
+    plain_ms, seeded_ms, infer_ms = [], [], []
     plain = seeded = None
     for k in range(args.warmup + args.steps):
-        plain = continuum.fit_arrays(test_in, order=1)
-        t = time.perf_counter()
-        y = net.predict(test_x)
-        infer = time.perf_counter() - t
+        plain = continuum.fit_arrays(test_in, order=1, tol=1e-15, tol_abs=1e-5)
+        y, inf_ms = net.predict(test_x, timing=True)
         a0, b0 = seed_mlp.seeds(y, test_scale)
-        seeded = continuum.fit_arrays(test_in, order=1, a0=a0, b0=b0)
+        seeded = continuum.fit_arrays(test_in, order=1, a0=a0, b0=b0, tol=1e-15, tol_abs=1e-5)
         if k >= args.warmup:
             plain_ms.append(plain["kernel_ms"])
             seeded_ms.append(seeded["kernel_ms"])
-            infer_s.append(infer)
-        log("step %d: plain fit %.2f ms, seeded fit %.2f ms, inference %.1f ms (wall)"
-            % (k, plain["kernel_ms"], seeded["kernel_ms"], 1e3 * infer))
-    both = (plain["status"] == 0) & (seeded["status"] == 0)
-    dchi = np.abs(seeded["chi2"][both] - plain["chi2"][both])
-    p_ms, s_ms = float(np.mean(plain_ms)), float(np.mean(seeded_ms))
+            infer_ms.append(inf_ms)
+        log("step %d: plain fit %.3f ms, seeded fit %.3f ms + inference %.3f ms (kernels)"
+            % (k, plain["kernel_ms"], seeded["kernel_ms"], inf_ms))
+    both = (plain["status"] == 0) & (seeded["status"] == 0) & (exact["status"] == 0)
+    dchi = np.abs(seeded["chi2"][both] - exact["chi2"][both])
+    dchi_plain = np.abs(plain["chi2"][both] - exact["chi2"][both])
+    p_ms = float(np.mean(plain_ms))
+    s_ms = float(np.mean(seeded_ms)) + float(np.mean(infer_ms))
     result = {
         "metric": "continuum fits/s (1 GPU), learned seed vs plain fit",
         "value": n / (s_ms / 1e3), "unit": "fits/s", "n_gpus": 1, "steps": args.steps,
@@ -448,14 +452,18 @@ def bench_continuum(args):
         "data": "synthetic",
         "config": {"workload": "continuum_%d" % n, "spectra": n, "training_spectra": n // 2,
                    "forest_pixels": int(test_in["offsets"][-1]),
-                   "plain_fit_ms": p_ms, "seeded_fit_ms": s_ms,
+                   "stopping": "EDM < 1e-5 (absolute) for both fits",
+                   "plain_fit_ms": p_ms, "seeded_fit_plus_inference_ms": s_ms,
+                   "inference_kernel_ms": float(np.mean(infer_ms)),
                    "plain_fits_per_s": n / (p_ms / 1e3),
-                   "seeded_speedup_fit_kernel": p_ms / s_ms,
-                   "inference_wall_ms": 1e3 * float(np.mean(infer_s)),
+                   "seeded_speedup": p_ms / s_ms,
+                   "exact_fit_ms": exact["kernel_ms"],
+                   "exact_mean_evaluations": float(exact["evaluations"].mean()),
                    "plain_mean_evaluations": float(plain["evaluations"].mean()),
                    "seeded_mean_evaluations": float(seeded["evaluations"].mean()),
                    "fits_valid_both": int(both.sum()),
-                   "chi2_max_abs_diff": float(dchi.max()) if dchi.size else 0.0,
+                   "chi2_seeded_minus_min_max": float(dchi.max()) if dchi.size else 0.0,
+                   "chi2_plain_minus_min_max": float(dchi_plain.max()) if dchi.size else 0.0,
                    "chi2_frac_within_1e-4": float((dchi < 1e-4).mean()) if dchi.size else 1.0,
                    "training_steps": int(len(losses)), "training_s": train_s,
                    "device": info["name"]},
@@ -469,7 +477,7 @@ def bench_continuum(args):
         res = [C.fit(S.forest_spec(test_in, k), order=1) for k in range(m)]
         cpu_s = time.time() - tc
         ref = np.array([r[2] for r in res])
-        rel = np.abs(plain["chi2"][:m] - ref) / np.abs(ref)
+        rel = np.abs(exact["chi2"][:m] - ref) / np.abs(ref)
         result["cpu_baseline"] = {
             "value": m / cpu_s, "unit": "fits/s", "cores": 1, "kind": "port",
             "sample": "first %d test spectra, FP64 numpy objective minimised by scipy "

@@ -184,6 +184,8 @@ typedef struct picca_cont_params {
     int32_t max_iter;             /* Levenberg-Marquardt iterations (e.g. 100)          */
     int32_t use_constant_weight;  /* 1: all weights 1 (chi2 = sum of squares)           */
     double tol;                   /* stop when g^T H^-1 g / 2 < tol * max(1, |chi2|)    */
+    double tol_abs;               /* or when g^T H^-1 g / 2 < tol_abs (0: off); Migrad's
+                                     own rule is 2e-4 at its default tolerance          */
 } picca_cont_params;
 
 int picca_cont_set_forests(picca_ctx *ctx, int64_t num_forests, const int64_t *offsets,
@@ -203,7 +205,8 @@ int picca_cont_fit(picca_ctx *ctx, const picca_cont_params *params, const double
  * accumulation; parameters, gradients and Adam state are FP32.
  *
  * picca_mlp_init sets the parameters (and resets Adam); picca_mlp_get reads them back;
- * picca_mlp_forward evaluates n rows of x [n][n_in] into y [n][2]; picca_mlp_gradients
+ * picca_mlp_forward evaluates n rows of x [n][n_in] into y [n][2] (kernel_ms, may be
+ * NULL, receives the device time of the forward kernel); picca_mlp_gradients
  * returns the gradients of loss = sum (y - t)^2 / (2 n) over one batch without updating;
  * picca_mlp_set_data makes a training set resident and picca_mlp_train runs `steps`
  * Adam steps (beta1 0.9, beta2 0.999, eps 1e-8) on minibatches of `batch` rows drawn
@@ -212,7 +215,7 @@ int picca_cont_fit(picca_ctx *ctx, const picca_cont_params *params, const double
 int picca_mlp_init(picca_ctx *ctx, int32_t n_in, int32_t n_hidden, const float *w1,
                    const float *b1, const float *w2, const float *b2);
 int picca_mlp_get(picca_ctx *ctx, float *w1, float *b1, float *w2, float *b2);
-int picca_mlp_forward(picca_ctx *ctx, int64_t n, const float *x, float *y);
+int picca_mlp_forward(picca_ctx *ctx, int64_t n, const float *x, float *y, double *kernel_ms);
 int picca_mlp_gradients(picca_ctx *ctx, int64_t n, const float *x, const float *t,
                         float *g_w1, float *g_b1, float *g_w2, float *g_b2, float *loss);
 int picca_mlp_set_data(picca_ctx *ctx, int64_t n, const float *x, const float *t);

@@ -76,7 +76,7 @@ _F64 = ctypes.c_double
 class ContParams(ctypes.Structure):
     """``picca_cont_params``."""
     _fields_ = [("max_iter", ctypes.c_int32), ("use_constant_weight", ctypes.c_int32),
-                ("tol", ctypes.c_double)]
+                ("tol", ctypes.c_double), ("tol_abs", ctypes.c_double)]
 
 
 _PROTOTYPES = {
@@ -97,7 +97,7 @@ _PROTOTYPES = {
     "picca_cont_fit": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "picca_mlp_init": [_P, _I32, _I32, _P, _P, _P, _P],
     "picca_mlp_get": [_P, _P, _P, _P, _P],
-    "picca_mlp_forward": [_P, _I64, _P, _P],
+    "picca_mlp_forward": [_P, _I64, _P, _P, _P],
     "picca_mlp_gradients": [_P, _I64, _P, _P, _P, _P, _P, _P, _P],
     "picca_mlp_set_data": [_P, _I64, _P, _P],
     "picca_mlp_train": [_P, _I64, _I32, ctypes.c_float, ctypes.c_uint64, _P],

@@ -32,7 +32,7 @@ def start_values(offsets, flux, ivar):
 
 
 def fit_arrays(inputs, order=1, use_constant_weight=False, a0=None, b0=None, max_iter=100,
-               tol=1e-12):
+               tol=1e-12, tol_abs=0.0):
     """Batched fit over flat per-pixel inputs (see ``synth_spectra.forest_inputs``).
 
     Returns the dict of ``Engine.cont_fit`` (a, b, chi2, ndata, status, evaluations,
@@ -48,7 +48,7 @@ def fit_arrays(inputs, order=1, use_constant_weight=False, a0=None, b0=None, max
         a0 = s0 if a0 is None else a0
         b0 = t0 if b0 is None else b0
     return eng.cont_fit(a0, b0, max_iter=max_iter, tol=tol,
-                        use_constant_weight=use_constant_weight)
+                        use_constant_weight=use_constant_weight, tol_abs=tol_abs)
 
 
 def _forest_arrays(forest, get_mean_cont, get_eta, get_var_lss, get_fudge, rest_frame_grid):

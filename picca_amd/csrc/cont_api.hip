@@ -62,8 +62,8 @@ int picca_cont_fit(picca_ctx *ctx, const picca_cont_params *params, const double
                    int32_t *status, int32_t *evaluations, double *kernel_ms) {
     if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
     if (!params) return fail(PICCA_E_VALUE, "params is NULL");
-    if (params->max_iter < 1 || !(params->tol > 0))
-        return fail(PICCA_E_VALUE, "max_iter must be >= 1 and tol > 0");
+    if (params->max_iter < 1 || !(params->tol > 0) || !(params->tol_abs >= 0))
+        return fail(PICCA_E_VALUE, "max_iter must be >= 1, tol > 0 and tol_abs >= 0");
     const int64_t nf = ctx->cont_forests;
     if (nf > 0 && (!a0 || !b0 || !a || !b || !chi2 || !ndata || !status || !evaluations))
         return fail(PICCA_E_VALUE, "arrays must not be NULL");
@@ -74,7 +74,8 @@ int picca_cont_fit(picca_ctx *ctx, const picca_cont_params *params, const double
     hipStream_t st = ctx->stream;
     PICCA_HIP_CHECK(ctx->c_a0.upload(a0, (size_t)nf, st));
     PICCA_HIP_CHECK(ctx->c_b0.upload(b0, (size_t)nf, st));
-    picca::ContConst K{params->max_iter, params->use_constant_weight, params->tol};
+    picca::ContConst K{params->max_iter, params->use_constant_weight, params->tol,
+                       params->tol_abs};
     picca::ContIn I{ctx->c_off.ptr,  ctx->c_flux.ptr, ctx->c_ivar.ptr,  ctx->c_mean.ptr,
                     ctx->c_x.ptr,    ctx->c_eta.ptr,  ctx->c_lss.ptr,   ctx->c_fudge.ptr,
                     ctx->c_order.ptr, ctx->c_a0.ptr,  ctx->c_b0.ptr};

@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void cont_fit_kernel(ContConst K, ContIn I, Co
         double na, nb;
         if (solve_step(s, free_b, 0.0, na, nb)) {
             const double edm = -0.5 * (s.ga * na + s.gb * nb);
-            if (edm < K.tol * fmax(1.0, fabs(s.chi))) {
+            if (edm < K.tol * fmax(1.0, fabs(s.chi)) || edm < K.tol_abs) {
                 status = 0;
                 break;
             }

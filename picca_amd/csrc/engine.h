@@ -211,7 +211,8 @@ size_t dmat_fill_lds(const DmatConst &C);
 struct ContConst {
     int32_t max_iter;
     int32_t use_constant_weight;
-    double tol;   // stop when g^T H^-1 g / 2 < tol * max(1, |chi2|)
+    double tol;      // stop when g^T H^-1 g / 2 < tol * max(1, |chi2|)
+    double tol_abs;  // or when g^T H^-1 g / 2 < tol_abs
 };
 
 struct ContIn {

@@ -115,10 +115,11 @@ int picca_mlp_get(picca_ctx *ctx, float *w1, float *b1, float *w2, float *b2) {
     return PICCA_OK;
 }
 
-int picca_mlp_forward(picca_ctx *ctx, int64_t n, const float *x, float *y) {
+int picca_mlp_forward(picca_ctx *ctx, int64_t n, const float *x, float *y, double *kernel_ms) {
     if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
     if (ctx->mlp_in == 0) return fail(PICCA_E_VALUE, "the network is not initialised");
     if (n < 0) return fail(PICCA_E_VALUE, "negative size");
+    if (kernel_ms) *kernel_ms = 0.0;
     if (n == 0) return PICCA_OK;
     if (!x || !y) return fail(PICCA_E_VALUE, "arrays must not be NULL");
     if (!finite(x, n * ctx->mlp_in)) return fail(PICCA_E_VALUE, "inputs must be finite");
@@ -127,9 +128,14 @@ int picca_mlp_forward(picca_ctx *ctx, int64_t n, const float *x, float *y) {
     DeviceBuffer<float> dx, dy;
     PICCA_HIP_CHECK(dx.upload(x, (size_t)(n * ctx->mlp_in), st));
     PICCA_HIP_CHECK(dy.reserve((size_t)(2 * n)));
+    PICCA_HIP_CHECK(hipEventRecord(ctx->ev_start, st));
     PICCA_HIP_CHECK(picca::launch_mlp_forward(mlp_view(ctx), dx.ptr, nullptr, n, nullptr, dy.ptr, st));
+    PICCA_HIP_CHECK(hipEventRecord(ctx->ev_stop, st));
     PICCA_HIP_CHECK(hipMemcpyAsync(y, dy.ptr, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, st));
     PICCA_HIP_CHECK(hipStreamSynchronize(st));
+    float ms = 0.f;
+    PICCA_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    if (kernel_ms) *kernel_ms = (double)ms;
     return PICCA_OK;
 }
 

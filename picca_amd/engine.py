@@ -185,7 +185,7 @@ class Engine:
               "picca_cont_set_forests")
         self._cont_forests = len(offsets) - 1
 
-    def cont_fit(self, a0, b0, max_iter=100, tol=1e-12, use_constant_weight=False):
+    def cont_fit(self, a0, b0, max_iter=100, tol=1e-12, use_constant_weight=False, tol_abs=0.0):
         """Fit every resident forest from (a0, b0).  Returns a dict of arrays a, b, chi2,
         ndata, status, evaluations and the kernel time kernel_ms."""
         n = self._cont_forests
@@ -193,7 +193,8 @@ class Engine:
         b0 = as_f64(b0)
         if len(a0) != n or len(b0) != n:
             raise ValueError("one start value per forest is needed")
-        params = _native.ContParams(int(max_iter), int(bool(use_constant_weight)), float(tol))
+        params = _native.ContParams(int(max_iter), int(bool(use_constant_weight)), float(tol),
+                                    float(tol_abs))
         out = {"a": np.zeros(n), "b": np.zeros(n), "chi2": np.zeros(n),
                "ndata": np.zeros(n, np.int32), "status": np.zeros(n, np.int32),
                "evaluations": np.zeros(n, np.int32)}

@@ -50,12 +50,14 @@ class SeedNetwork:
         check(self._lib.picca_mlp_get(self._eng._ctx, *(ptr(p) for p in out)), "picca_mlp_get")
         return out
 
-    def predict(self, x):
+    def predict(self, x, timing=False):
+        """Network outputs [n, 2]; with ``timing`` also the forward kernel's device ms."""
         x = _f32(x)
         y = np.zeros((x.shape[0], 2), np.float32)
-        check(self._lib.picca_mlp_forward(self._eng._ctx, x.shape[0], ptr(x), ptr(y)),
-              "picca_mlp_forward")
-        return y
+        ms = ctypes.c_double()
+        check(self._lib.picca_mlp_forward(self._eng._ctx, x.shape[0], ptr(x), ptr(y),
+                                          ctypes.byref(ms)), "picca_mlp_forward")
+        return (y, ms.value) if timing else y
 
     def gradients(self, x, t):
         x, t = _f32(x), _f32(t)

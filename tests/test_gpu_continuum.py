@@ -2,7 +2,8 @@
 
 Bar: for every forest the GPU minimum chi2 is no higher than the oracle's and within
 1e-9 relative of it (both sit at the minimum of the same FP64 objective), (a, b) within
-1e-6 relative of the oracle's, ndata and the validity verdict identical;
+1e-5 relative of the oracle's (the valley of chi2 fixes them only to about sqrt(tol)),
+ndata and the validity verdict identical;
 use_constant_weight fits equal the closed-form least squares to 1e-9.
 """
 import numpy as np
@@ -34,9 +35,9 @@ def test_fit_matches_oracle(inputs, order):
         assert res["ndata"][k] == ndata
         assert res["chi2"][k] <= chi + 1e-12 * abs(chi)
         assert res["chi2"][k] == pytest.approx(chi, rel=1e-9)
-        assert res["a"][k] == pytest.approx(a, rel=1e-6)
+        assert res["a"][k] == pytest.approx(a, rel=1e-5)
         scale = max(abs(a), 1e-12)
-        assert abs(res["b"][k] - b) <= 1e-6 * scale
+        assert abs(res["b"][k] - b) <= 1e-5 * scale
         if order == 0:
             assert res["b"][k] == 0.0
 
@@ -60,7 +61,9 @@ def test_starts_in_the_basin_reach_the_same_minimum(inputs):
     b0 = base["b"] + 0.2 * np.abs(base["a"]) * rng.uniform(-1, 1, n)
     res = continuum.fit_arrays(inputs, order=1, a0=a0, b0=b0)
     np.testing.assert_allclose(res["chi2"], base["chi2"], rtol=1e-10)
-    np.testing.assert_allclose(res["a"], base["a"], rtol=1e-6)
+    # chi2 pins the minimum; along its flattest direction (a, b) are fixed only to about
+    # sqrt(tol) of their scale
+    np.testing.assert_allclose(res["a"], base["a"], rtol=1e-5)
 
 
 def test_negative_and_zero_continuum_are_flagged():
