@@ -12,6 +12,12 @@
  *                      py/picca/cf.py:250-387 runs on the GPU for every listed pair
  *   picca_dmat_*       compute_dmat          py/picca/cf.py:390-517 and its per-pair
  *                      kernel compute_dmat_forest_pairs_fast py/picca/cf.py:520-887
+ *   picca_xcf_*        xcf.compute_xi        py/picca/xcf.py:131-220 and its per-forest
+ *                      kernel py/picca/xcf.py:223-322
+ *   picca_cont_*       compute_continuum     py/picca/delta_extraction/expected_fluxes/
+ *                      utils.py:12-136 (the Migrad fit of its chi2), batched over forests
+ *   picca_neighbour_candidates  the healpy query_disc superset of fill_neighs
+ *                      py/picca/cf.py:82-135 and py/picca/xcf.py:71-128
  *   picca_cf_params    the module globals    py/picca/cf.py:28-79 (None -> has_* = 0)
  */
 #ifndef PICCA_HIP_H
@@ -152,7 +158,9 @@ int picca_xcf_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_re
  *   weights_dmat[np*nt], dmat[np*nt * nm] (nm = model np*nt), r_par_eff[nm],
  *   r_trans_eff[nm], z_eff[nm], weight_eff[nm].
  * stats (may be NULL) receives: [0] in-range pairs, [1] selected pairs,
- * [2] in-range pairs beyond the count-pass size (see py/picca/cf.py:619,702-703).
+ * [2] in-range pairs beyond the count-pass size (see py/picca/cf.py:619,702-703),
+ * [3] selected pairs whose data bin reaches np*nt by rounding (the reference writes those
+ * past the end of weights_dmat; here they are counted and dropped).  stats has 4 entries.
  */
 int picca_dmat_run(picca_ctx *ctx, const picca_cf_params *params, int64_t num_pairs,
                    const int32_t *pair_a, const int32_t *pair_b, const double *pair_ang,
