@@ -43,6 +43,7 @@ int build_xi_const(const picca_cf_params *p, picca::XiConst &C, int &mode) {
     C.np = p->num_bins_r_par;
     C.nt = p->num_bins_r_trans;
     C.nb = C.np * C.nt;
+    C.ntp = C.nt + 1;
     C.rpmin = p->r_par_min;
     C.rpmax = p->r_par_max;
     C.rtmax = p->r_trans_max;
@@ -50,6 +51,7 @@ int build_xi_const(const picca_cf_params *p, picca::XiConst &C, int &mode) {
     C.np_d = (double)C.np;
     C.nt_d = (double)C.nt;
     C.nb_d = (double)C.nb;
+    C.ntp_d = (double)C.ntp;
     C.inv_p = C.np_d / C.span_p;
     C.inv_t = C.nt_d / C.rtmax;
     C.width_p = C.span_p / C.np;
@@ -319,11 +321,12 @@ int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_rep
     int mode = 0;
     int status = build_xi_const(params, C, mode);
     if (status != PICCA_OK) return status;
-    // The fast loop classifies pairs from cos*r1 - cos*r2 and sin*d1 + sin*d2 instead of
-    // (r1 - r2)*cos and (d1 + d2)*sin; both differ from the reference value by a few ulp
-    // of the largest coordinate.  Widen the margins around every bin edge and range limit
-    // by that bound (in bin units) so that every pair whose bin or range test could differ
-    // is re-evaluated exactly.
+    // The fast loop takes the bin coordinates as |r1 - r2| * (cos * np / span) and
+    // (d1 + d2) * (sin * nt / r_trans_max) instead of the reference's
+    // ((r1 - r2) * cos - r_par_min) / span * np and (d1 + d2) * sin / r_trans_max * nt; both
+    // differ from the exact value by a few ulp of the coordinate.  The margins around every
+    // bin edge and range limit are widened by a bound of that difference (in bin units), so
+    // that every pair whose bin or range test could differ is re-evaluated exactly.
     {
         const double u = std::ldexp(1.0, -53);
         const double err_p = 16.0 * u * ctx->max_abs_r * C.inv_p + 8.0 * u * (C.np_d + 1.0);
@@ -335,31 +338,41 @@ int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_rep
         if (!(C.half_p > 0.25 && C.half_t > 0.25))
             return fail(PICCA_E_VALUE, "edge_tol is too large compared with the bin width");
     }
-    // LDS: histogram (48 B per bin) when it fits, plus one staging tile per wave
+    // the specialised kernel: standard binning, auto-correlation, r_par_min == 0, no cuts
+    const bool general = mode != picca::MODE_STANDARD || params->x_correlation ||
+                         params->r_par_min != 0.0 || params->has_z_min_pairs ||
+                         params->has_z_max_pairs || params->has_zerr_cut ||
+                         params->remove_same_half_plate_close_pairs;
+    // LDS: the histogram in its padded layout ((np + 1) x (nt + 1) bins of 48 B) when it
+    // fits, plus one staging tile per wave for the loops that stage pixels (the fast loop
+    // of the specialised kernel stages none)
     const size_t static_lds = 1024;
     const size_t waves = picca::XI_BLOCK / 64;
+    const size_t hist_bytes = (size_t)(C.np + 1) * (size_t)C.ntp * picca::XI_HIST_BIN_BYTES;
     auto lds_bytes = [&](bool with_hist, int tile) {
-        return (with_hist ? (size_t)C.nb * picca::XI_HIST_BIN_BYTES : 0) +
-               waves * (size_t)(tile + 1) * picca::XI_TILE_PIXEL_BYTES;
+        return (with_hist ? hist_bytes : 0) + waves * (size_t)(tile + 1) * picca::XI_TILE_PIXEL_BYTES;
     };
     bool lds_hist = false;
     size_t lds = 0;
-    for (int tile = 64; tile >= 16 && !lds_hist; tile /= 2) {
-        if ((int64_t)(lds_bytes(true, tile) + static_lds) <= ctx->lds_limit) {
+    if (!general) {
+        if ((int64_t)(lds_bytes(true, 0) + static_lds) <= ctx->lds_limit) {
             lds_hist = true;
-            C.tile_len = tile;
-            lds = lds_bytes(true, tile);
+            C.tile_len = 0;
+            lds = lds_bytes(true, 0);
+        }
+    } else {
+        for (int tile = 64; tile >= 16 && !lds_hist; tile /= 2) {
+            if ((int64_t)(lds_bytes(true, tile) + static_lds) <= ctx->lds_limit) {
+                lds_hist = true;
+                C.tile_len = tile;
+                lds = lds_bytes(true, tile);
+            }
         }
     }
     if (!lds_hist) {  // histogram too large for LDS: runs go straight to the HBM rows
         C.tile_len = 64;
         lds = lds_bytes(false, 64);
     }
-    // the specialised kernel: standard binning, auto-correlation, r_par_min == 0, no cuts
-    const bool general = mode != picca::MODE_STANDARD || params->x_correlation ||
-                         params->r_par_min != 0.0 || params->has_z_min_pairs ||
-                         params->has_z_max_pairs || params->has_zerr_cut ||
-                         params->remove_same_half_plate_close_pairs;
     PICCA_HIP_CHECK(hipSetDevice(ctx->device));
     const int64_t nrows = ctx->num_rows;
     const int64_t nout = nrows * C.nb;

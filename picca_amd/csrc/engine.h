@@ -38,7 +38,8 @@ enum : int {
 struct XiConst {
     int32_t np, nt, nb, flags;
     int32_t tile_len;       // delta2 pixels staged per wave and tile (<= 64)
-    int32_t pad_;
+    int32_t ntp;            // nt + 1: row stride of the LDS histogram of the xi kernel,
+                            // whose last column and last row collect rejected pairs
     double rpmin, rpmax, rtmax;
     double span_p;          // r_par_max - r_par_min
     double inv_p, inv_t;    // np / span_p, nt / r_trans_max
@@ -51,6 +52,7 @@ struct XiConst {
     double zerr_ang;        // zerr_cut_deg * pi / 180
     double zerr_kms;
     double np_d, nt_d, nb_d;
+    double ntp_d;           // nt + 1
 };
 
 // One pixel as the pair kernel streams it through the scalar cache.

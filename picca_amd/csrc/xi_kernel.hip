@@ -6,31 +6,36 @@
 //
 // Work decomposition
 //   * one persistent workgroup of 1024 threads (16 wave64) per CU keeps the histogram of
-//     its current output row in LDS, 48 B per bin (5 FP64 sums + u32 count, 120 KB at
-//     50x50 bins), and adds it to the row in HBM with FP64 atomics when the row changes.
-//     When np*nt does not fit in LDS the runs are added straight to the row in HBM;
+//     its current output row in LDS, 48 B per bin (5 FP64 sums + u32 count), and adds it to
+//     the row in HBM with FP64 atomics when the row changes.  The LDS histogram is padded
+//     to (np + 1) x (nt + 1) bins (125 KB at 50x50): the extra row and column collect the
+//     pairs the range test rejects, so the fast loop bins every pair without a branch.
+//     When it does not fit in LDS the runs are added straight to the row in HBM;
 //   * workgroups pull "groups" (<= 64 consecutive pairs of one row) from an HBM queue;
 //   * inside a group, waves pull tasks = (pair, chunk of 64 pixels of delta1) from an
 //     LDS counter.  Lane l owns pixel i = 64*chunk + l of delta1 in registers; the wave
 //     walks the pixels j of delta2 in lockstep;
-//   * pixels j are staged per wave through LDS: lane l loads pixel j0 + l (coalesced),
-//     pixels the reference skips before any arithmetic (w2 == 0, zerr cut) are dropped,
-//     the rest compacted, and the next tile is already loading while the wave walks the
-//     current one with wave-uniform (broadcast) LDS reads;
+//   * specialised kernel (standard binning, auto-correlation, no cuts): pixel j is a
+//     wave-uniform record read through the scalar cache;
+//     general kernel: pixels j are staged per wave through LDS (lane l loads pixel j0 + l,
+//     pixels the reference skips before any arithmetic are dropped, the rest compacted,
+//     the next tile already loading) and read back with broadcast LDS reads;
 //   * for the standard (r_par, r_trans) binning the j range is pruned to the window where
 //     |r_par| < r_par_max and r_trans < r_trans_max can hold for some lane (r_comov and
 //     dist_m are non-decreasing along a forest; forests where they are not are walked in
 //     full).  The skipped pixel pairs are provably rejected by the reference's range test;
-//   * a lane accumulates consecutive accepted pairs that land in the same bin in
-//     registers (r_par and r_trans move monotonically with j) and adds the run to the
-//     histogram with six LDS atomics when its bin changes.
+//   * a lane accumulates consecutive pairs that land in the same bin in registers
+//     (r_par and r_trans move monotonically with j) and adds the run to the histogram
+//     with six LDS atomics when its bin changes.
 //
-// Arithmetic: r_par, r_trans and the bin index use the reference's FP64 operations in the
-// reference's order (built with -ffp-contract=off).  The bin index is first taken with a
-// reciprocal multiply; whenever the fractional part lies within the edge tolerance (plus
-// a rounding margin) of an integer it is recomputed with the reference's
-// divide-then-multiply, so every pair gets the reference's bin.  The mean pair redshift
-// (z1 + z2) / 2 is formed as z1/2 + z2/2, which is the same double.
+// Arithmetic: r_par, r_trans and the bin index follow the reference's FP64 operations
+// (built with -ffp-contract=off).  The bin index is first taken with a reciprocal
+// multiply; whenever the fractional part lies within the edge tolerance (plus a rounding
+// margin) of an integer it is recomputed with the reference's divide-then-multiply, so
+// every pair gets the reference's bin.  The mean pair redshift (z1 + z2) / 2 is formed as
+// z1/2 + z2/2, which is the same double.  The fast loop sums |r1 - r2| w1 w2 and
+// (d1 + d2) w1 w2 and multiplies by cos(ang/2), sin(ang/2) once per run, which changes
+// the r_par / r_trans sums by rounding only (relative 1e-16 per pair).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -44,13 +49,13 @@ namespace picca {
 // zerr skips, with the reference's arithmetic (py/picca/cf.py:336-387): pair-redshift cut,
 // separations, range test, bins (reciprocal multiply, exact divide-then-multiply near an
 // edge), same-half-plate cut, overflow guard.  Returns true and sets bin / r_par / r_trans
-// for an accepted pair.
+// for an accepted pair; pbin is the same bin in the padded LDS layout (row stride nt + 1).
 template <int MODE, bool XCORR, bool GENERAL>
 __device__ __forceinline__ bool exact_pair(const XiConst &C, double r1, double d1, double r2,
                                            double d2, double z, double ang, double cos_half,
                                            double sin_half, bool same_half_plate, int &bin,
-                                           double &r_par, double &r_trans, uint32_t &edge,
-                                           uint32_t &overflow) {
+                                           int &pbin, double &r_par, double &r_trans,
+                                           uint32_t &edge, uint32_t &overflow) {
     if (GENERAL) {
         if ((C.flags & XI_HAS_ZMIN) && z < C.zmin) return false;
         if ((C.flags & XI_HAS_ZMAX) && z > C.zmax) return false;
@@ -98,17 +103,103 @@ __device__ __forceinline__ bool exact_pair(const XiConst &C, double r1, double d
         return false;
     }
     bin = (int)bin_f;
+    // a pair whose r_trans bin rounds to nt lands in bin 0 of the next r_par row, as in the
+    // reference's flat index
+    pbin = (int)(f_trans + C.ntp_d * f_par) + (f_trans >= C.nt_d ? 1 : 0);
     return true;
 }
+
+// The exact re-evaluation of exact_pair for the fast loop (standard binning,
+// auto-correlation, r_par_min == 0, no cuts) as straight-line code with every constant
+// read once at the top: the fast loop runs it for a handful of pairs per billion, and a
+// branchy version leaves loads outstanding on some of its paths, which the pixel loop
+// would then have to wait for.  Returns the padded bin, or `trash` for a rejected pair;
+// edge and overflow pairs are counted in LDS.
+__device__ __forceinline__ int exact_fast(const XiConst &K, double r1, double d1, double r2,
+                                          double d2, double cos_half, double sin_half,
+                                          int trash, __attribute__((address_space(3)))
+                                          unsigned long long *lds_stats) {
+    const double rpmax = K.rpmax, rtmax = K.rtmax, tol = K.edge_tol, span = K.span_p;
+    const double np_d = K.np_d, nt_d = K.nt_d, nb_d = K.nb_d, ntp_d = K.ntp_d;
+    const double width_p = K.width_p, width_t = K.width_t;
+    const double r_par = fabs((r1 - r2) * cos_half);
+    const double r_trans = (d1 + d2) * sin_half;
+    const bool out = (r_par >= rpmax) | (r_trans >= rtmax);
+    const bool edge_out = ((r_par >= rpmax) & (r_par - rpmax < tol)) |
+                          ((r_trans >= rtmax) & (r_trans - rtmax < tol));
+    const double t_par = (r_par - 0.0) / span * np_d;
+    const double t_trans = r_trans / rtmax * nt_d;
+    const double f_par = floor(t_par);
+    const double f_trans = floor(t_trans);
+    const bool edge_in = ((r_par != 0) & (edge_distance(t_par, width_p) < tol)) |
+                         ((r_trans != 0) & (edge_distance(t_trans, width_t) < tol));
+    const double bin_f = f_trans + nt_d * f_par;
+    const bool over = !((bin_f >= 0.0) & (bin_f < nb_d));
+    const bool edge = out ? edge_out : edge_in;
+    if (edge)
+        __hip_atomic_fetch_add(&lds_stats[XI_STAT_EDGE], 1ull, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (!out & over)
+        __hip_atomic_fetch_add(&lds_stats[XI_STAT_OVERFLOW], 1ull, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    // f_par, f_trans are small non-negative integers unless the pair is rejected
+    const double pbin_f = f_trans + ntp_d * f_par + (f_trans >= nt_d ? 1.0 : 0.0);
+    return (out | over) ? trash : (int)fmin(pbin_f, (double)trash);
+}
+
+// Run of the fast loop: consecutive pairs of one lane in one padded bin.  r_par and r_trans
+// are summed before their cos(ang/2) / sin(ang/2) factors, which are constant over a task
+// and applied when the run is added to the histogram.
+struct FastRun {
+    int32_t bin;
+    double count;  // exact below 2^53
+    double we, xi, rp, rt, z;
+};
+
+__device__ __forceinline__ void flush_fast(const FastRun &run, LocalBin *hist, double cos_half,
+                                           double sin_half) {
+    LocalBin *b = (LocalBin *)((LocalByte *)hist +
+                               __umul24((uint32_t)run.bin, (uint32_t)sizeof(LdsBin)));
+    __hip_atomic_fetch_add(&b->we, run.we, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&b->xi, run.xi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&b->rp, run.rp * cos_half, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&b->rt, run.rt * sin_half, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&b->z, run.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&b->nb, (uint32_t)run.count, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Wave-uniform test of a pixel weight for +0 or -0 on the scalar unit (a compiler-chosen
+// FP64 compare would take a vector instruction per pixel).
+__device__ __forceinline__ bool uniform_zero_weight(double w) {
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(w);
+    unsigned long long twice;
+    asm volatile("s_lshl_b64 %0, %1, 1" : "=s"(twice) : "s"(bits) : "scc");
+    return twice == 0;
+}
+
+// min of two finite doubles as one v_min_f64 (fmin would first canonicalise an operand
+// that the compiler cannot prove canonical, once per call)
+__device__ __forceinline__ double min_f64(double a, double b) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "s"(b));
+    return r;
+}
+
+typedef __attribute__((address_space(3))) unsigned long long LocalU64;
 
 // One task: pixels [64*chunk, 64*chunk + 64) of delta1 against the pruned window of delta2.
 //   GENERAL = false: standard binning, auto-correlation, r_par_min == 0, no z / zerr /
 //   same-half-plate cuts (the cuts of the reference that cannot trigger are compiled out).
-template <int MODE, bool XCORR, bool GENERAL, typename Bin>
+//   LDS_HIST: the histogram is the padded LDS layout (row stride nt + 1), otherwise the
+//   output row in HBM.
+template <int MODE, bool XCORR, bool GENERAL, bool LDS_HIST, typename Bin>
 __device__ void xi_task(const XiConst &C, const XiConst &C_lds, const ForestView &F,
                         const PairView &P, int64_t p, int chunk, int lane, Bin *hist,
                         TilePixel *tile, TileExact *tile_x, LaneCounters &cnt,
-                        uint64_t &n_steps) {
+                        LocalU64 *lds_stats, uint64_t &n_steps) {
     const int a = uniform_int(P.a[p]);
     const int b = uniform_int(P.b[p]);
     const int64_t oa = uniform_i64(F.off[a]);
@@ -182,49 +273,70 @@ __device__ void xi_task(const XiConst &C, const XiConst &C_lds, const ForestView
     j_lo = uniform_int(j_lo);
     j_hi = uniform_int(j_hi);
 
-    LaneRun run;
-    run.bin = -1;
-    run.count = 0;
-    run.we = run.xi = run.rp = run.rt = run.z = 0.0;
-    uint32_t visit = 0;
-    uint32_t edge = 0;
-    uint32_t overflow = 0;
-    const int tile_len = C.tile_len;
-
-    if (!GENERAL) {
+    if constexpr (!GENERAL && LDS_HIST) {
         // Pixel j of delta2 is wave-uniform: its record comes through the scalar cache into
         // SGPRs and the loop touches LDS only to flush runs.  Two pixels per iteration, the
         // next two already loading; the store is padded, so reading up to three records
         // past j_hi stays in bounds.
+        //
+        // Every pair of an active lane takes a bin of the padded histogram: a pair whose
+        // r_par or r_trans lies beyond the range clamps to row np or column nt, which
+        // collect the rejected pairs and are never read.  Inactive lanes are moved far
+        // beyond r_par_max so that their (zero-weight) pairs always land there.
         const PixelRecord *rec = F.pix + ob;
+        const double cp = cos_half * C.inv_p;  // bins per Mpc/h of |r1 - r2|
+        const double st = sin_half * C.inv_t;  // bins per Mpc/h of d1 + d2
+        const double cap_p = C.np_d;  // clamps of the bin coordinates: row np, column nt
+        const double cap_t = C.nt_d;
+        const int trash = (C.np + 1) * C.ntp - 1;
+        const double r1f = active ? r1 : 1e300;
         uint32_t steps = 0;
+        FastRun run;
+        run.bin = trash;
+        run.count = 0.0;
+        run.we = run.xi = run.rp = run.rt = run.z = 0.0;
         PixelRecord c0 = load_record(rec + j_lo), c1 = load_record(rec + j_lo + 1);
+        // consume the first two records here, so that their loads are waited for before
+        // the loop and the loop only ever waits for records it prefetched an iteration
+        // earlier
+        asm volatile("" ::"s"(c0.w), "s"(c0.z), "s"(c1.w), "s"(c1.z));
         auto step = [&](const PixelRecord &q) {
             // the reference skips w2 == 0 (either sign) before any arithmetic
-            if ((__double_as_longlong(q.w) << 1) == 0) return;
+            if (uniform_zero_weight(q.w)) return;
             ++steps;
-            double r_par = (r1 - q.r) * cos_half;
-            const double r_trans = (d1 + q.d) * sin_half;
-            const double t_par = fabs(r_par) * C.inv_p;
-            const double t_trans = r_trans * C.inv_t;
-            const double f_par = floor(t_par);
-            const double f_trans = floor(t_trans);
-            // r_par == 0 exactly (equal r_comov) is bin 0 and never an edge pair
-            const bool near = ((fabs(t_par - f_par - 0.5) > C.half_p) & (r_par != 0)) |
-                              (fabs(t_trans - f_trans - 0.5) > C.half_t);
-            bool accept = active & !near & (t_par < C.np_d) & (t_trans < C.nt_d);
-            int bin = (int)__umul24((unsigned)C.nt, (unsigned)(int)f_par) + (int)f_trans;
-            double rp = r_par, rt = r_trans;
+            const double diff = r1f - q.r;
+            const double sum = d1 + q.d;
+            // |r_par| and r_trans in bin units.  Away from an edge the bin is the integer
+            // part; within the rounding margin of an edge (or of the range limits, which
+            // are edges) the pair is redone with the reference's arithmetic.  r_par == 0
+            // exactly (equal r_comov) is bin 0 and never an edge pair.
+            const double tp = fabs(diff) * cp;
+            const double tt = sum * st;
+            const double fp = __builtin_amdgcn_fract(tp);
+            const double ft = __builtin_amdgcn_fract(tt);
+            const bool near = ((fabs(fp - 0.5) > C.half_p) & (diff != 0)) |
+                              (fabs(ft - 0.5) > C.half_t);
+            int bin = (int)__umul24((unsigned)C.ntp, (unsigned)(int)min_f64(tp, cap_p)) +
+                      (int)min_f64(tt, cap_t);
             if (active & near) {
-                // exact re-evaluation (py/picca/cf.py:355-376)
-                // (constants from the LDS copy: this rare path keeps no SGPRs busy)
-                accept = exact_pair<MODE, XCORR, false>(C_lds, r1, d1, q.r, q.d, 0.0, ang,
-                                                        cos_half, sin_half, false, bin, rp, rt,
-                                                        edge, overflow);
+                // exact re-evaluation (py/picca/cf.py:355-376), constants from the LDS copy
+                bin = exact_fast(C_lds, r1, d1, q.r, q.d, cos_half, sin_half, trash, lds_stats);
             }
-            if (accept)
-                add_pair(run, hist, cnt, bin, w1 * q.w, dw1 * q.dw, fabs(rp), rt,
-                         fma(q.z, 0.5, zh1));
+            const double w12 = w1 * q.w;
+            const double dw12 = dw1 * q.dw;
+            const double z = fma(q.z, 0.5, zh1);
+            // a new bin adds the lane's run to the histogram and restarts it: the restart is
+            // a multiply by 0 (every input is finite), which costs no branch of its own
+            const bool change = bin != run.bin;
+            if (change) flush_fast(run, hist, cos_half, sin_half);
+            const double keep = change ? 0.0 : 1.0;
+            run.bin = bin;
+            run.count = fma(run.count, keep, 1.0);
+            run.we = fma(run.we, keep, w12);
+            run.xi = fma(run.xi, keep, dw12);
+            run.rp = fma(fabs(diff), w12, run.rp * keep);
+            run.rt = fma(sum, w12, run.rt * keep);
+            run.z = fma(z, w12, run.z * keep);
         };
         for (int j = j_lo; j < j_hi; j += 2) {
             const PixelRecord n0 = load_record(rec + j + 2), n1 = load_record(rec + j + 3);
@@ -233,57 +345,24 @@ __device__ void xi_task(const XiConst &C, const XiConst &C_lds, const ForestView
             c0 = n0;
             c1 = n1;
         }
+        flush_fast(run, hist, cos_half, sin_half);
         n_steps += steps;
-        if (active) visit += steps;
-        if (run.bin >= 0) {
-            flush_run(run, hist);
-            cnt.accept += run.count;
-        }
-        cnt.visit += visit;
-        cnt.edge += edge;
-        cnt.overflow += overflow;
+        if (active) cnt.visit += steps;
         return;
-    }
+    } else {
+        LaneRun run;
+        run.bin = -1;
+        run.count = 0;
+        run.we = run.xi = run.rp = run.rt = run.z = 0.0;
+        uint32_t visit = 0;
+        uint32_t edge = 0;
+        uint32_t overflow = 0;
+        const int tile_len = C.tile_len;
 
-    double pr = 0, pd = 0, pz = 0, pw = 0, pdw = 0;
-    bool pvalid = false;
-    {
-        const int j = j_lo + lane;
-        pvalid = lane < tile_len && j < j_hi;
-        if (pvalid) {
-            pr = Br[j];
-            pd = Bd[j];
-            pz = Bz[j];
-            pw = Bw[j];
-            pdw = Bdw[j];
-        }
-    }
-    for (int j0 = j_lo; j0 < j_hi; j0 += tile_len) {
-        // stage the prefetched tile: drop what the reference skips, compact the rest
-        bool keep = pvalid && pw != 0;
-        if (GENERAL && zerr_on && keep) {
-            double z_qf = 0.5 * (pz + z_qso_1);
-            double dv_kms = fabs(pz - z_qso_1) / (1 + z_qf);
-            dv_kms *= SPEED_LIGHT_KMS;
-            if (dv_kms < C.zerr_kms) keep = false;
-        }
-        const uint64_t mask = __ballot(keep);
-        const int n_tile = uniform_int(__popcll(mask));
-        const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
-        if (keep) {
-            TilePixel *t = tile + pos;
-            t->cr = cos_half * pr;
-            t->sd = sin_half * pd;
-            t->w = pw;
-            t->dw = pdw;
-            t->zh = pz * 0.5;
-            tile_x[pos].r = pr;
-            tile_x[pos].d = pd;
-        }
-        __builtin_amdgcn_wave_barrier();
+        double pr = 0, pd = 0, pz = 0, pw = 0, pdw = 0;
+        bool pvalid = false;
         {
-            const int j = j0 + tile_len + lane;
+            const int j = j_lo + lane;
             pvalid = lane < tile_len && j < j_hi;
             if (pvalid) {
                 pr = Br[j];
@@ -293,8 +372,42 @@ __device__ void xi_task(const XiConst &C, const XiConst &C_lds, const ForestView
                 pdw = Bdw[j];
             }
         }
-        n_steps += n_tile;
-        {
+        for (int j0 = j_lo; j0 < j_hi; j0 += tile_len) {
+            // stage the prefetched tile: drop what the reference skips, compact the rest
+            bool keep = pvalid && pw != 0;
+            if (GENERAL && zerr_on && keep) {
+                double z_qf = 0.5 * (pz + z_qso_1);
+                double dv_kms = fabs(pz - z_qso_1) / (1 + z_qf);
+                dv_kms *= SPEED_LIGHT_KMS;
+                if (dv_kms < C.zerr_kms) keep = false;
+            }
+            const uint64_t mask = __ballot(keep);
+            const int n_tile = uniform_int(__popcll(mask));
+            const int pos = __builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+            if (keep) {
+                TilePixel *t = tile + pos;
+                t->cr = cos_half * pr;
+                t->sd = sin_half * pd;
+                t->w = pw;
+                t->dw = pdw;
+                t->zh = pz * 0.5;
+                tile_x[pos].r = pr;
+                tile_x[pos].d = pd;
+            }
+            __builtin_amdgcn_wave_barrier();
+            {
+                const int j = j0 + tile_len + lane;
+                pvalid = lane < tile_len && j < j_hi;
+                if (pvalid) {
+                    pr = Br[j];
+                    pd = Bd[j];
+                    pz = Bz[j];
+                    pw = Bw[j];
+                    pdw = Bdw[j];
+                }
+            }
+            n_steps += n_tile;
             for (int u = 0; u < n_tile; ++u) {
                 const double2 wdw = *reinterpret_cast<const double2 *>(&tile[u].w);
                 const double zh2 = tile[u].zh;
@@ -304,32 +417,56 @@ __device__ void xi_task(const XiConst &C, const XiConst &C_lds, const ForestView
                 if (!((C.flags & XI_HAS_ZMIN) && z < C.zmin) &&
                     !((C.flags & XI_HAS_ZMAX) && z > C.zmax))
                     ++visit;
-                int bin;
+                int bin, pbin;
                 double r_par, r_trans;
                 if (exact_pair<MODE, XCORR, GENERAL>(C, r1, d1, rd.x, rd.y, z, ang, cos_half,
-                                                     sin_half, same_half_plate, bin, r_par,
-                                                     r_trans, edge, overflow))
-                    add_pair(run, hist, cnt, bin, w1 * wdw.x, dw1 * wdw.y, r_par, r_trans, z);
+                                                     sin_half, same_half_plate, bin, pbin,
+                                                     r_par, r_trans, edge, overflow))
+                    add_pair(run, hist, cnt, LDS_HIST ? pbin : bin, w1 * wdw.x, dw1 * wdw.y,
+                             r_par, r_trans, z);
             }
+            __builtin_amdgcn_wave_barrier();
         }
-        __builtin_amdgcn_wave_barrier();
+        if (run.bin >= 0) {
+            flush_run(run, hist);
+            cnt.accept += run.count;
+        }
+        cnt.visit += visit;
+        cnt.edge += edge;
+        cnt.overflow += overflow;
     }
-    if (run.bin >= 0) {
-        flush_run(run, hist);
-        cnt.accept += run.count;
+}
+
+// Adds the real bins of the padded LDS histogram to output row `row` in HBM and clears
+// them; returns the pairs this thread moved (the accepted-pair statistic).
+__device__ __forceinline__ uint64_t flush_row(const XiConst &C, LocalBin *hist, XiBin *out,
+                                              int tid) {
+    uint64_t moved = 0;
+    for (int k = tid; k < C.nb; k += XI_BLOCK) {
+        const int kp = k + k / C.nt;  // padded index: row k / nt, column k % nt
+        const uint32_t n = hist[kp].nb;
+        if (n) {
+            unsafeAtomicAdd(&out[k].we, hist[kp].we);
+            unsafeAtomicAdd(&out[k].xi, hist[kp].xi);
+            unsafeAtomicAdd(&out[k].rp, hist[kp].rp);
+            unsafeAtomicAdd(&out[k].rt, hist[kp].rt);
+            unsafeAtomicAdd(&out[k].z, hist[kp].z);
+            atomicAdd(&out[k].nb, (unsigned long long)n);
+            hist[kp].we = hist[kp].xi = hist[kp].rp = hist[kp].rt = hist[kp].z = 0.0;
+            hist[kp].nb = 0;
+            moved += n;
+        }
     }
-    cnt.visit += visit;
-    cnt.edge += edge;
-    cnt.overflow += overflow;
+    return moved;
 }
 
 template <int MODE, bool XCORR, bool GENERAL, bool LDS_HIST>
 __global__ void __launch_bounds__(XI_BLOCK)
 xi_kernel(XiConst C, ForestView F, PairView P, GroupView G, XiOut O) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int nb = C.nb;
+    const int nbp = (C.np + 1) * C.ntp;  // padded LDS histogram
     LocalBin *hist = (LocalBin *)reinterpret_cast<LdsBin *>(smem);
-    const size_t hist_bytes = LDS_HIST ? (size_t)nb * sizeof(LdsBin) : 0;
+    const size_t hist_bytes = LDS_HIST ? (size_t)nbp * sizeof(LdsBin) : 0;
     // per wave: (tile_len + 1) staged pixels, then (tile_len + 1) exact coordinates
     const int wave = threadIdx.x >> 6;
     unsigned char *wave_tiles = smem + hist_bytes +
@@ -343,12 +480,13 @@ xi_kernel(XiConst C, ForestView F, PairView P, GroupView G, XiOut O) {
     __shared__ int s_prefix[XI_GROUP_PAIRS + 1];
     __shared__ unsigned long long s_stats[XI_NSTATS];
     __shared__ XiConst s_const;
+    LocalU64 *lds_stats = (LocalU64 *)s_stats;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
 
     if (LDS_HIST) {
-        for (int k = tid; k < nb; k += XI_BLOCK) {
+        for (int k = tid; k < nbp; k += XI_BLOCK) {
             hist[k].we = hist[k].xi = hist[k].rp = hist[k].rt = hist[k].z = 0.0;
             hist[k].nb = 0;
         }
@@ -357,6 +495,7 @@ xi_kernel(XiConst C, ForestView F, PairView P, GroupView G, XiOut O) {
     if (tid == 0) s_const = C;  // published by the first __syncthreads of the group loop
 
     LaneCounters cnt = {0, 0, 0, 0};
+    uint64_t accepted = 0;  // LDS_HIST: pairs moved from the histogram to HBM
     uint64_t n_steps = 0, n_tasks = 0, n_empty_tasks = 0;  // wave-uniform
     int current_row = -1;
 
@@ -367,24 +506,8 @@ xi_kernel(XiConst C, ForestView F, PairView P, GroupView G, XiOut O) {
         const int64_t g = s_group;
         if (g >= G.count) break;
         const int row = G.row[g];
-        if (LDS_HIST && row != current_row) {
-            if (current_row >= 0) {
-                XiBin *out = O.bins + (int64_t)current_row * nb;
-                for (int k = tid; k < nb; k += XI_BLOCK) {
-                    const uint32_t n = hist[k].nb;
-                    if (n) {
-                        unsafeAtomicAdd(&out[k].we, hist[k].we);
-                        unsafeAtomicAdd(&out[k].xi, hist[k].xi);
-                        unsafeAtomicAdd(&out[k].rp, hist[k].rp);
-                        unsafeAtomicAdd(&out[k].rt, hist[k].rt);
-                        unsafeAtomicAdd(&out[k].z, hist[k].z);
-                        atomicAdd(&out[k].nb, (unsigned long long)n);
-                        hist[k].we = hist[k].xi = hist[k].rp = hist[k].rt = hist[k].z = 0.0;
-                        hist[k].nb = 0;
-                    }
-                }
-            }
-        }
+        if (LDS_HIST && row != current_row && current_row >= 0)
+            accepted += flush_row(C, hist, O.bins + (int64_t)current_row * C.nb, tid);
         current_row = row;
         const int64_t p0 = G.begin[g];
         const int npairs = (int)(G.begin[g + 1] - p0);
@@ -426,37 +549,26 @@ xi_kernel(XiConst C, ForestView F, PairView P, GroupView G, XiOut O) {
             const int chunk = uniform_int(t - s_prefix[q]);
             const uint64_t steps_before = n_steps;
             if (LDS_HIST) {
-                xi_task<MODE, XCORR, GENERAL>(C, s_const, F, P, p0 + q, chunk, lane, hist, tile,
-                                              tile_x, cnt, n_steps);
+                xi_task<MODE, XCORR, GENERAL, true>(C, s_const, F, P, p0 + q, chunk, lane, hist,
+                                                    tile, tile_x, cnt, lds_stats, n_steps);
             } else {
-                xi_task<MODE, XCORR, GENERAL>(C, s_const, F, P, p0 + q, chunk, lane,
-                                              O.bins + (int64_t)row * nb, tile, tile_x, cnt,
-                                              n_steps);
+                xi_task<MODE, XCORR, GENERAL, false>(C, s_const, F, P, p0 + q, chunk, lane,
+                                                     O.bins + (int64_t)row * C.nb, tile, tile_x,
+                                                     cnt, lds_stats, n_steps);
             }
             ++n_tasks;
             if (n_steps == steps_before) ++n_empty_tasks;
         }
     }
     // final flush (all waves are past the last __syncthreads of the loop)
-    if (LDS_HIST && current_row >= 0) {
-        XiBin *out = O.bins + (int64_t)current_row * nb;
-        for (int k = tid; k < nb; k += XI_BLOCK) {
-            const uint32_t n = hist[k].nb;
-            if (n) {
-                unsafeAtomicAdd(&out[k].we, hist[k].we);
-                unsafeAtomicAdd(&out[k].xi, hist[k].xi);
-                unsafeAtomicAdd(&out[k].rp, hist[k].rp);
-                unsafeAtomicAdd(&out[k].rt, hist[k].rt);
-                unsafeAtomicAdd(&out[k].z, hist[k].z);
-                atomicAdd(&out[k].nb, (unsigned long long)n);
-            }
-        }
-    }
+    if (LDS_HIST && current_row >= 0)
+        accepted += flush_row(C, hist, O.bins + (int64_t)current_row * C.nb, tid);
+    if (!LDS_HIST) accepted = cnt.accept;
     // statistics: lane values -> wave sums -> workgroup -> HBM
     const uint64_t v_visit = wave_sum_u64(cnt.visit);
     const uint64_t v_edge = wave_sum_u64(cnt.edge);
     const uint64_t v_over = wave_sum_u64(cnt.overflow);
-    const uint64_t v_accept = wave_sum_u64(cnt.accept);
+    const uint64_t v_accept = wave_sum_u64(accepted);
     if (lane == 0) {
         atomicAdd(&s_stats[XI_STAT_VISITED], (unsigned long long)v_visit);
         atomicAdd(&s_stats[XI_STAT_EDGE], (unsigned long long)v_edge);
