@@ -40,6 +40,9 @@ extern "C" {
 
 /* pair flag bits (picca_xi_set_pairs / picca_dmat_run: pair_flags) */
 #define PICCA_PAIR_SAME_HALF_PLATE 1u
+/* picca_dmat_run: a forest x quasar pair, whose count pass rejects r_par <= r_par_min
+   (py/picca/xcf.py:464-466) */
+#define PICCA_PAIR_XCF_COUNT 2u
 
 /*
  * Configuration of one correlation run: the cf module globals (py/picca/cf.py:28-79)
@@ -157,6 +160,11 @@ int picca_xcf_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_re
  * Outputs are accumulated into the caller's arrays:
  *   weights_dmat[np*nt], dmat[np*nt * nm] (nm = model np*nt), r_par_eff[nm],
  *   r_trans_eff[nm], z_eff[nm], weight_eff[nm].
+ * A forest with order < 0 in the store is not projected: the cross-correlation
+ * distortion matrix (compute_dmat of py/picca/xcf.py:324-658) stores each kept quasar
+ * as a one-pixel forest (r_comov, dist_m, z_qso, weight) with order -1 and lists the
+ * (delta, quasar) pairs with PICCA_PAIR_XCF_COUNT and x_correlation = 1; alpha2 is then
+ * the quasars' redshift-evolution exponent.
  * stats (may be NULL) receives: [0] in-range pairs, [1] selected pairs,
  * [2] in-range pairs beyond the count-pass size (see py/picca/cf.py:619,702-703),
  * [3] selected pairs whose data bin reaches np*nt by rounding (the reference writes those

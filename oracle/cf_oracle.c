@@ -13,6 +13,9 @@
  *                        py/picca/bin/picca_cf.py:452-458)
  *   oracle_dmat_rows  <- the loop of compute_dmat over kept pairs,
  *                        py/picca/cf.py:425-502
+ *   oracle_xcf_pair   <- compute_xi_forest_pairs_fast of xcf, py/picca/xcf.py:223-322
+ *   oracle_xdmat_forest <- compute_dmat_forest_pairs_fast of xcf,
+ *                        py/picca/xcf.py:427-658 (one delta with its kept quasars)
  *
  * Loop order, branch order and every floating-point operation follow the reference
  * (divide-then-multiply bin assignment, cos/sin of ang/2 from libm, sequential sums
@@ -539,4 +542,184 @@ int oracle_dmat_rows(const oracle_params *p, const int64_t *offsets,
         if (status != 0) return status;
     }
     return 0;
+}
+
+
+/*
+ * The cross-correlation distortion matrix of one delta against its kept quasars
+ * (compute_dmat_forest_pairs_fast of py/picca/xcf.py:427-658).  Quasar j has r_comov2[j],
+ * dist_m2[j], z2[j] (its redshift), weights2[j] and angle ang[j].  Only the delta's
+ * continuum fit distorts the field: eta2 (w1 / sum w1) and, for order1 == 1, eta4
+ * (w1 dll1 / sum w1 dll1^2) per quasar and model bin.  The redshift-evolution factor uses
+ * alpha for the pixel and alpha2 (the quasars' exponent) for the quasar.
+ * Returns 0, -1 for a selected pair with a negative data bin (IndexError in the
+ * reference), -2 on allocation failure.  stats as oracle_dmat_pair: [0] in-range pairs of
+ * the main pass, [1] selected pairs, [2] in-range pairs beyond the count-pass size (the
+ * count pass rejects r_par <= r_par_min, the main pass only r_par < r_par_min,
+ * py/picca/xcf.py:464-466, 542-544), [3] selected pairs with a data bin >= np*nt.
+ */
+int oracle_xdmat_forest(const oracle_params *p, const double *log_lambda1,
+                        const double *r_comov1, const double *dist_m1, const double *z1,
+                        const double *weights1, int64_t n1, const double *r_comov2,
+                        const double *dist_m2, const double *z2, const double *weights2,
+                        const double *ang, int64_t n2, double *weights_dmat, double *dmat,
+                        double *r_par_eff, double *r_trans_eff, double *z_eff,
+                        double *weight_eff, int32_t order1, int64_t *stats) {
+    const int64_t n_model = (int64_t)p->num_model_bins_r_par * p->num_model_bins_r_trans;
+    const int64_t nb = (int64_t)p->num_bins_r_par * p->num_bins_r_trans;
+
+    /* (i) count pass */
+    int64_t num_pairs = 0;
+    for (int64_t i = 0; i < n1; ++i) {
+        if (weights1[i] == 0) continue;
+        for (int64_t j = 0; j < n2; ++j) {
+            if (weights2[j] == 0) continue;
+            double r_par = (r_comov1[i] - r_comov2[j]) * cos(ang[j] / 2);
+            double r_trans = (dist_m1[i] + dist_m2[j]) * sin(ang[j] / 2);
+            if (p->rmu_binning) {
+                r_trans = sqrt(r_trans * r_trans + r_par * r_par);
+                r_par /= r_trans;
+            }
+            if (r_par >= p->r_par_max || r_trans >= p->r_trans_max || r_par <= p->r_par_min)
+                continue;
+            num_pairs += 1;
+        }
+    }
+    if (num_pairs == 0) return 0;
+
+    /* (ii) per-forest sums, sequential like numba's ndarray.sum */
+    double sum_weights1 = 0;
+    for (int64_t i = 0; i < n1; ++i) sum_weights1 += weights1[i];
+    double acc1 = 0;
+    for (int64_t i = 0; i < n1; ++i) acc1 += log_lambda1[i] * weights1[i];
+    const double mean_log_lambda1 = acc1 / sum_weights1;
+    double *dll1 = (double *)malloc(sizeof(double) * (size_t)(n1 > 0 ? n1 : 1));
+    double *eta2 = (double *)calloc((size_t)(n_model * n2), sizeof(double));
+    double *eta4 = (double *)calloc((size_t)(n_model * n2), sizeof(double));
+    /* the main pass can find more in-range pairs than the count pass: size the records
+       by an upper bound and keep the reference's count-pass-many model bins for unique */
+    const int64_t cap = n1 * n2 > 0 ? n1 * n2 : 1;
+    int32_t *sel_data = (int32_t *)malloc(sizeof(int32_t) * (size_t)cap);
+    int32_t *sel_model = (int32_t *)calloc((size_t)cap, sizeof(int32_t));
+    int32_t *sel_i = (int32_t *)calloc((size_t)cap, sizeof(int32_t));
+    int32_t *sel_j = (int32_t *)calloc((size_t)cap, sizeof(int32_t));
+    int32_t *all_model = (int32_t *)calloc((size_t)num_pairs, sizeof(int32_t));
+    int32_t *uniq = (int32_t *)malloc(sizeof(int32_t) * (size_t)num_pairs);
+    int status = 0;
+    if (!dll1 || !eta2 || !eta4 || !sel_data || !sel_model || !sel_i || !sel_j ||
+        !all_model || !uniq) {
+        status = -2;
+        goto done;
+    }
+    for (int64_t i = 0; i < n1; ++i) dll1[i] = log_lambda1[i] - mean_log_lambda1;
+    double sw_dll1 = 0;
+    for (int64_t i = 0; i < n1; ++i) sw_dll1 += weights1[i] * (dll1[i] * dll1[i]);
+
+    /* (iii) main pass */
+    int64_t counter_selected = 0, counter_pairs = 0;
+    for (int64_t i = 0; i < n1; ++i) {
+        if (weights1[i] == 0) continue;
+        for (int64_t j = 0; j < n2; ++j) {
+            if (weights2[j] == 0) continue;
+            double z = (z1[i] + z2[j]) / 2;
+            int j_selected = 1;
+            if ((p->has_z_min_pairs && z < p->z_min_pairs) ||
+                (p->has_z_max_pairs && z > p->z_max_pairs))
+                j_selected = 0;
+            double r_par = (r_comov1[i] - r_comov2[j]) * cos(ang[j] / 2);
+            double r_trans = (dist_m1[i] + dist_m2[j]) * sin(ang[j] / 2);
+            if (p->rmu_binning) {
+                r_trans = sqrt(r_trans * r_trans + r_par * r_par);
+                r_par /= r_trans;
+            }
+            if (r_par >= p->r_par_max || r_trans >= p->r_trans_max || r_par < p->r_par_min)
+                continue;
+            double weights12 = weights1[i] * weights2[j];
+            double zfac;
+            if (p->redshift_evolution) {
+                zfac = pow((1 + z1[i]) / (1 + p->z_ref), p->alpha - 1) *
+                       pow((1 + z2[j]) / (1 + p->z_ref), p->alpha2 - 1);
+            } else {
+                zfac = 1;
+            }
+            double bins_r_par = floor((r_par - p->r_par_min) / (p->r_par_max - p->r_par_min) * p->num_bins_r_par);
+            double bins_r_trans = floor(r_trans / p->r_trans_max * p->num_bins_r_trans);
+            int32_t bins = (int32_t)(bins_r_trans + p->num_bins_r_trans * bins_r_par);
+            double model_bins_r_par = floor((r_par - p->r_par_min) / (p->r_par_max - p->r_par_min) * p->num_model_bins_r_par);
+            double model_bins_r_trans = floor(r_trans / p->r_trans_max * p->num_model_bins_r_trans);
+            int32_t model_bins = (int32_t)(model_bins_r_trans + p->num_model_bins_r_trans * model_bins_r_par);
+
+            if (counter_pairs < num_pairs) all_model[counter_pairs] = model_bins;
+            else stats[2] += 1;
+            counter_pairs += 1;
+            stats[0] += 1;
+
+            if (model_bins < 0 || model_bins >= n_model) continue;  /* out of bounds in numba */
+            if (j_selected) {
+                sel_model[counter_selected] = model_bins;
+                sel_data[counter_selected] = bins;
+                sel_i[counter_selected] = (int32_t)i;
+                sel_j[counter_selected] = (int32_t)j;
+                counter_selected += 1;
+                stats[1] += 1;
+                r_par_eff[model_bins] += weights12 * r_par;
+                r_trans_eff[model_bins] += weights12 * r_trans;
+                z_eff[model_bins] += weights12 * z;
+                weight_eff[model_bins] += weights12;
+                if (bins < nb) {
+                    if (bins >= 0) weights_dmat[bins] += weights12;
+                } else {
+                    stats[3] += 1;
+                }
+            }
+            eta2[j + n2 * model_bins] += zfac * weights1[i] / sum_weights1;
+            if (order1 == 1)
+                eta4[j + n2 * model_bins] += zfac * (weights1[i] * dll1[i] / sw_dll1);
+        }
+    }
+
+    /* (iv) fill over the unique model bins of the first num_pairs in-range pairs */
+    {
+        memcpy(uniq, all_model, sizeof(int32_t) * (size_t)num_pairs);
+        qsort(uniq, (size_t)num_pairs, sizeof(int32_t), cmp_i32);
+        int64_t n_uniq = 0;
+        for (int64_t k = 0; k < num_pairs; ++k)
+            if (k == 0 || uniq[k] != uniq[n_uniq - 1]) uniq[n_uniq++] = uniq[k];
+        for (int64_t pair = 0; pair < counter_selected; ++pair) {
+            int32_t i = sel_i[pair], j = sel_j[pair];
+            int32_t bins = sel_data[pair];
+            if (bins < 0) {
+                status = -1;
+                goto done;
+            }
+            int32_t model_bins = sel_model[pair];
+            double weights12 = weights1[i] * weights2[j];
+            double zfac;
+            if (p->redshift_evolution) {
+                zfac = pow((1 + z1[i]) / (1 + p->z_ref), p->alpha - 1) *
+                       pow((1 + z2[j]) / (1 + p->z_ref), p->alpha2 - 1);
+            } else {
+                zfac = 1;
+            }
+            if (bins >= nb) continue;
+            dmat[model_bins + n_model * (int64_t)bins] += zfac * weights12;
+            for (int64_t u = 0; u < n_uniq; ++u) {
+                int32_t k = uniq[u];
+                if (k < 0 || k >= n_model) continue;
+                dmat[k + n_model * (int64_t)bins] +=
+                    weights12 * (-eta2[j + n2 * k] - eta4[j + n2 * k] * dll1[i]);
+            }
+        }
+    }
+done:
+    free(dll1);
+    free(eta2);
+    free(eta4);
+    free(sel_data);
+    free(sel_model);
+    free(sel_i);
+    free(sel_j);
+    free(all_model);
+    free(uniq);
+    return status;
 }

@@ -139,6 +139,10 @@ def lib():
                                          + [ctypes.c_int64] + [P] * 5 + [ctypes.c_int64]
                                          + [P] * 7)
         _lib.oracle_xcf_pair.restype = None
+        _lib.oracle_xdmat_forest.argtypes = ([ctypes.POINTER(OracleParams)] + [P] * 5
+                                             + [ctypes.c_int64] + [P] * 5 + [ctypes.c_int64]
+                                             + [P] * 6 + [ctypes.c_int32, P])
+        _lib.oracle_xdmat_forest.restype = ctypes.c_int
     return _lib
 
 
@@ -424,3 +428,49 @@ def compute_xcf(params, data, healpixs, neighbours):
     for arr in (xi, rp, rt, z):
         arr[w] /= we[w]
     return (we, xi, rp, rt, z, npairs), stats
+
+
+def xdmat_forest(params, ll1, r1, d1, z1, w1, r2, d2, z2, w2, ang, out, order1, stats):
+    """compute_dmat_forest_pairs_fast of the cross-correlation (py/picca/xcf.py:427-658):
+    one delta against its kept quasars; ``out`` = (weights_dmat, dmat (flat), r_par_eff,
+    r_trans_eff, z_eff, weight_eff) updated in place."""
+    a = [_f(x) for x in (ll1, r1, d1, z1, w1)]
+    b = [_f(x) for x in (r2, d2, z2, w2, ang)]
+    status = lib().oracle_xdmat_forest(ctypes.byref(params), *[_p(x) for x in a], len(a[0]),
+                                       *[_p(x) for x in b], len(b[0]), *[_p(x) for x in out],
+                                       int(order1), _p(stats))
+    if status == -1:
+        raise IndexError("negative bin index")
+    if status != 0:
+        raise MemoryError("oracle_xdmat_forest failed")
+
+
+def compute_xdmat(params, data, healpixs, neighbours, reject):
+    """compute_dmat of the cross-correlation (py/picca/xcf.py:324-424) with the legacy
+    global numpy stream: per delta, one draw per quasar neighbour, a delta with no kept
+    quasar is skipped before its pairs are counted.  ``params.alpha2`` is the quasars'
+    redshift-evolution exponent.  Returns the compute_dmat tuple and stats[4]."""
+    nb = params.num_bins_r_par * params.num_bins_r_trans
+    nm = params.num_model_bins_r_par * params.num_model_bins_r_trans
+    out = [np.zeros(nb), np.zeros(nb * nm)] + [np.zeros(nm) for _ in range(4)]
+    stats = np.zeros(4, dtype=np.int64)
+    num_pairs = 0
+    num_pairs_used = 0
+    for hp in healpixs:
+        for d1 in data[hp]:
+            if d1.order is None:
+                raise RuntimeError("order is not defined for the deltas")
+            objs_n, ang = neighbours[id(d1)]
+            w = np.random.rand(len(objs_n)) > reject
+            if w.sum() == 0:
+                continue
+            num_pairs += len(objs_n)
+            num_pairs_used += int(w.sum())
+            kept = [o for o, k in zip(objs_n, w) if k]
+            ang_k = np.asarray(ang)[w]
+            xdmat_forest(params, d1.log_lambda, d1.r_comov, d1.dist_m, d1.z, d1.weights,
+                         [o.r_comov for o in kept], [o.dist_m for o in kept],
+                         [o.z_qso for o in kept], [o.weights for o in kept], ang_k, out,
+                         d1.order, stats)
+    out[1] = out[1].reshape(nb, nm)
+    return (out[0], out[1], out[2], out[3], out[4], out[5], num_pairs, num_pairs_used), stats

@@ -18,6 +18,9 @@ PICCA_E_INDEX = -3
 PICCA_E_HIP = -4
 PICCA_E_NOMEM = -5
 
+PICCA_PAIR_SAME_HALF_PLATE = 1
+PICCA_PAIR_XCF_COUNT = 2
+
 PAIR_SAME_HALF_PLATE = 1
 
 

@@ -777,7 +777,7 @@ int picca_dmat_run(picca_ctx *ctx, const picca_cf_params *params, int64_t num_pa
                                    hipMemcpyDeviceToHost, ctx->stream));
     PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     if (h_stats[picca::DMAT_STAT_NEGATIVE] > 0)
-        return fail(PICCA_E_VALUE, "a selected pair has a negative data bin index");
+        return fail(PICCA_E_INDEX, "a selected pair has a negative data bin index");
     for (int64_t k = 0; k < nb; ++k) weights_dmat[k] += h_wdm[k];
     parallel_for(nb * nm, [&](int64_t lo, int64_t hi) {
         for (int64_t k = lo; k < hi; ++k) dmat[k] += h_dmat[k];

@@ -26,6 +26,12 @@
 //                     eta1, eta3 (Q, Q', eta2, eta4) built by walking that pixel's pairs,
 //                     followed by their outer products; one atomic add per block entry.
 // All arithmetic is FP64; bins follow the reference's divide-then-multiply.
+//
+// The forest x quasar distortion matrix (compute_dmat_forest_pairs_fast of
+// py/picca/xcf.py:427-658) runs through the same kernels: the quasar is a one-pixel
+// delta2 with order < 0, which removes every term of delta2's continuum fit (eta1, eta3,
+// eta5..eta8) and leaves the identity, eta2 (w1 / sum w1) and eta4 terms of the
+// reference; its pair flag PAIR_XCF_COUNT gives the count pass of xcf.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <limits.h>
@@ -45,6 +51,8 @@ struct DmatPair {
     bool shp_cut;   // remove_same_half_plate_close_pairs and the pair is on one half plate
     bool zerr_on;   // zerr cut active for this pair (ang < zerr_cut_deg)
     bool order1, order2;
+    bool proj2;     // delta2 is projected (order >= 0); a quasar is not (order < 0)
+    bool strict;    // cross-correlation count pass: r_par == r_par_min is not counted
     bool prune;     // standard binning, both forests monotone, cos(ang/2) > 0
 };
 
@@ -71,6 +79,8 @@ __device__ DmatPair load_pair(const DmatConst &C, const DmatForests &F, const Dm
     q.zerr_on = (C.flags & XI_HAS_ZERR) && (q.ang < C.zerr_ang);
     q.order1 = uniform_int(F.order[q.a]) == 1;
     q.order2 = uniform_int(F.order[q.b]) == 1;
+    q.proj2 = uniform_int(F.order[q.b]) >= 0;
+    q.strict = (uniform_int(P.flags[s]) & PAIR_XCF_COUNT) != 0;
     q.prune = !(C.flags & DMAT_RMU) && q.cos_half > 0 &&
               (uniform_int(F.mono[q.a]) & 3) == 3 && (uniform_int(F.mono[q.b]) & 3) == 3;
     return q;
@@ -292,7 +302,9 @@ __global__ __launch_bounds__(DMAT_PASS_BLOCK) void dmat_pass_kernel(DmatConst C,
                 ++n_in;
                 ++cnt_in_i;
                 const bool close = q.shp_cut && fabs(r_par) < C.width_p;
-                if (!close) ++n_cp;
+                // the forest x quasar count pass rejects r_par <= r_par_min
+                // (py/picca/xcf.py:464-466)
+                if (!close && !(q.strict && r_par <= C.rpmin)) ++n_cp;
                 if (M < 0 || M >= C.nm) continue;
                 const double z = (z1 + z2) / 2;
                 if (!q.shp_cut && M != last_m) {
@@ -518,10 +530,10 @@ __global__ __launch_bounds__(DMAT_PASS_BLOCK) void dmat_pass_kernel(DmatConst C,
                     if (rm.key >= 0) flush_m(rm);
                     rm = RunM{M, 0, 0, 0, 0, 0, 0, 0, 0};
                 }
-                rm.e5 += zfac * w12 / q.sum_w1 / q.sum_w2;
+                if (q.proj2) rm.e5 += zfac * w12 / q.sum_w1 / q.sum_w2;
                 if (q.order2) rm.e6 += zfac * w1 / q.sum_w1 * (w2 * l2 / q.sw2);
                 if (q.order1) {
-                    rm.e7 += zfac * w2 / q.sum_w2 * a7;
+                    if (q.proj2) rm.e7 += zfac * w2 / q.sum_w2 * a7;
                     if (q.order2) rm.e8 += zfac * w1 * l1 * w2 * l2 / q.sw1 / q.sw2;
                 }
                 if (!sel) continue;
@@ -734,7 +746,7 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
                                     const int mu = M % C.ntm;
                                     const int m = M / C.ntm - mrp0;
                                     const bool in_m = mu == u && m >= 0 && m < nmrp;
-                                    if (in_m && mask[m] != 0) {
+                                    if (in_m && mask[m] != 0 && q.proj2) {
                                         lds_add(vec + VEC_E + m, zfac * w2 / q.sum_w2);
                                         if (q.order2)
                                             lds_add(vec + VEC_E2 + m, zfac * w2 * l2 / q.sw2);

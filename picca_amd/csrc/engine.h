@@ -21,7 +21,7 @@ enum : int {
     XI_PRUNE = 16,
 };
 
-enum : int { PAIR_SAME_HALF_PLATE = 1 };
+enum : int { PAIR_SAME_HALF_PLATE = 1, PAIR_XCF_COUNT = 2 };
 
 enum : int {
     XI_STAT_VISITED = 0,

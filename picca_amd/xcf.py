@@ -10,7 +10,12 @@ Module globals with picca's names (py/picca/xcf.py:20-67), and:
   healpix pixels in one GPU launch, normalised like the reference;
 * ``compute_xi_rows``: one output row per healpix in one launch (what picca_xcf needs);
 * ``compute_xi_forest_pairs_fast``: the in-place per-forest kernel (:223-322), as a
-  launch of one forest.
+  launch of one forest;
+* ``compute_dmat`` (:324-424): the distortion matrix of the kept (delta, quasar) pairs
+  of the given healpix pixels in one batched GPU computation, and its per-forest kernel
+  ``compute_dmat_forest_pairs_fast`` (:427-658).  Each quasar enters the engine's forest
+  store as a one-pixel forest with order -1 (not projected), so the distortion-matrix
+  kernels of the auto-correlation keep only the terms of the delta's continuum fit.
 
 The neighbour candidates come from the engine's grid search; the angles are
 ``QSO.get_angle_between`` evaluated with numpy exactly as the reference does.
@@ -20,6 +25,7 @@ import sys
 import numpy as np
 
 from . import engine as _engine
+from . import _native
 from .constants import SPEED_LIGHT
 from .neighbours import find_neighbours
 from .store import ForestStore
@@ -115,6 +121,35 @@ def _catalogue():
 
 def current_params():
     return _engine.params_from_globals(sys.modules[__name__])
+
+
+def dmat_params():
+    """The globals as distortion-matrix parameters: signed r_par, and the quasars'
+    redshift-evolution exponent alpha_obj in the second-forest slot."""
+    params = current_params()
+    params.x_correlation = 1
+    params.alpha2 = float(alpha_obj) if alpha_obj is not None else float("nan")
+    return params
+
+
+def _quasar_forests(store, obj):
+    """The deltas' store followed by one one-pixel forest per quasar (order -1)."""
+    nf = len(store.offsets) - 1
+    nq = len(obj.objects)
+    offsets = np.concatenate([store.offsets,
+                              store.offsets[-1] + np.arange(1, nq + 1, dtype=np.int64)])
+    ll = store.log_lambda if store.log_lambda is not None else np.zeros(len(store.z))
+    return ForestStore(
+        offsets, np.concatenate([store.z, obj.z_qso]),
+        np.concatenate([store.r_comov, obj.r_comov]),
+        np.concatenate([store.dist_m, obj.dist_m]),
+        np.concatenate([store.weights, obj.weights]),
+        np.concatenate([store.delta, np.zeros(nq)]),
+        np.concatenate([store.z_qso, obj.z_qso]),
+        np.concatenate([store.ra, obj.ra]), np.concatenate([store.dec, obj.dec]),
+        np.concatenate([store.thingid, obj.thingid]),
+        log_lambda=np.concatenate([ll, np.zeros(nq)]),
+        order=np.concatenate([store.order, np.full(nq, -1, dtype=np.int32)])), nf
 
 
 def fill_neighs(healpixs):
@@ -256,3 +291,91 @@ def compute_xi_forest_pairs_fast(z1, r_comov1, dist_m1, weights1, delta1, z2, r_
     rebin_r_trans += res.r_trans[0]
     rebin_z += res.z[0]
     rebin_num_pairs += res.num_pairs[0]
+
+
+_dmat_store_cache = None
+
+
+def _dmat_store(cat):
+    global _dmat_store_cache
+    if _dmat_store_cache is None or _dmat_store_cache[0] != cat.key:
+        store, nf = _quasar_forests(cat.store, cat.objects)
+        _dmat_store_cache = (cat.key, store, nf)
+    return _dmat_store_cache[1], _dmat_store_cache[2]
+
+
+def build_dmat_pairs(healpixs):
+    """Kept (delta, quasar) pairs of compute_dmat (py/picca/xcf.py:357-407) in reference
+    order: one draw of numpy's global legacy stream per quasar neighbour of each delta; a
+    delta that keeps no quasar is skipped before its pairs are counted and keeps its
+    neighbour list.  Returns (forest index, quasar index, angle, num_pairs,
+    num_pairs_used)."""
+    cat = _catalogue()
+    fa, qb, angs = [], [], []
+    num_pairs = 0
+    num_pairs_used = 0
+    for healpix in healpixs:
+        for delta1 in data[healpix]:
+            if delta1.order is None:
+                raise RuntimeError(
+                    "Trying to compute the distortion matrix but order is not defined for "
+                    "the deltas. Check previous warning to solve this issue")
+            idx = getattr(delta1, "neighbour_index", None)
+            n = 0 if idx is None or delta1.neighbours is None else len(idx)
+            keep = np.random.rand(n) > reject
+            if keep.sum() == 0:
+                continue
+            num_pairs += n
+            num_pairs_used += int(keep.sum())
+            fa.append(np.full(int(keep.sum()), cat.index_of(delta1), dtype=np.int64))
+            qb.append(np.asarray(idx, dtype=np.int64)[keep])
+            angs.append(np.asarray(delta1.neighbour_angles, dtype=np.float64)[keep])
+            delta1.neighbours = None
+    cat_arrays = [np.concatenate(x) if x else np.zeros(0) for x in (fa, qb, angs)]
+    return (cat_arrays[0].astype(np.int64), cat_arrays[1].astype(np.int64),
+            cat_arrays[2].astype(np.float64), num_pairs, num_pairs_used)
+
+
+def compute_dmat(healpixs):
+    """compute_dmat (py/picca/xcf.py:324-424) over the deltas of ``healpixs``; consumes
+    their neighbour lists (fill_neighs first).  Returns (weights_dmat, dmat [nb, nm],
+    r_par_eff, r_trans_eff, z_eff, weight_eff, num_pairs, num_pairs_used)."""
+    global last_report
+    cat = _catalogue()
+    fa, qb, ang, num_pairs, num_pairs_used = build_dmat_pairs(healpixs)
+    store, nf = _dmat_store(cat)
+    eng = _engine.get_engine(device)
+    eng.set_forests(store, key=("xdmat",) + cat.key)
+    flags = np.full(len(fa), _native.PICCA_PAIR_XCF_COUNT, dtype=np.uint8)
+    (weights_dmat, dmat, r_par_eff, r_trans_eff, z_eff, weight_eff,
+     stats) = eng.dmat(dmat_params(), fa, nf + qb, ang, flags)
+    last_report = {"in_range_pairs": int(stats[0]), "selected_pairs": int(stats[1]),
+                   "beyond_count_pass": int(stats[2]), "overflow_pairs": int(stats[3])}
+    return (weights_dmat, dmat, r_par_eff, r_trans_eff, z_eff, weight_eff, num_pairs,
+            num_pairs_used)
+
+
+def compute_dmat_forest_pairs_fast(log_lambda1, r_comov1, dist_m1, z1, weights1, r_comov2,
+                                   dist_m2, z2, weights2, ang, weights_dmat, dmat, r_par_eff,
+                                   r_trans_eff, z_eff, weight_eff, order1):
+    """Adds one delta against its kept quasars to the caller's arrays
+    (py/picca/xcf.py:427-658); runs as one GPU computation."""
+    n1 = len(z1)
+    n2 = len(z2)
+    offsets = np.concatenate([[0, n1], n1 + np.arange(1, n2 + 1)]).astype(np.int64)
+    store = ForestStore(
+        offsets, np.concatenate([z1, z2]), np.concatenate([r_comov1, r_comov2]),
+        np.concatenate([dist_m1, dist_m2]), np.concatenate([weights1, weights2]),
+        np.zeros(n1 + n2), np.concatenate([[0.0], z2]), np.zeros(n2 + 1), np.zeros(n2 + 1),
+        np.arange(n2 + 1), log_lambda=np.concatenate([log_lambda1, np.zeros(n2)]),
+        order=np.concatenate([[order1], np.full(n2, -1)]))
+    eng = _engine.get_engine(device)
+    eng.set_forests(store)
+    out = eng.dmat(dmat_params(), np.zeros(n2, np.int64), 1 + np.arange(n2), ang,
+                   np.full(n2, _native.PICCA_PAIR_XCF_COUNT, dtype=np.uint8))
+    weights_dmat += out[0]
+    dmat += out[1].reshape(dmat.shape)
+    r_par_eff += out[2]
+    r_trans_eff += out[3]
+    z_eff += out[4]
+    weight_eff += out[5]

@@ -223,6 +223,32 @@ class XcfSetup:
         self.ang_max = utils.compute_ang_max(cosmo, self.r_trans_max, z_min, z_min2)
 
 
+XDMAT_CASES = {
+    "xdmat": XCF_COMMON + ["--rej", "0.99", "--no-redshift-evolution"],
+    "xdmat_zcuts": XCF_COMMON + ["--rej", "0.99", "--z-min-pairs", "2.25",
+                                 "--z-max-pairs", "2.3", "--z-min-sources", "2.3",
+                                 "--z-max-sources", "2.5", "--no-redshift-evolution"],
+}
+
+
+class XdmatSetup(XcfSetup):
+    """Globals, deltas and quasars of one picca_xdmat test command
+    (py/picca/bin/picca_xdmat.py:338-459; the mean-delta removal of XcfSetup does not
+    change a distortion matrix, which reads no delta values)."""
+
+    def __init__(self, argv):
+        argv = list(argv)
+        self.redshift_evolution = "--no-redshift-evolution" not in argv
+        argv = [a for a in argv if a != "--no-redshift-evolution"]
+        opts = dict(zip(argv[0::2], argv[1::2]))
+        super().__init__(argv)
+        self.reject = float(opts["--rej"])
+        self.num_model_bins_r_par = self.num_bins_r_par
+        self.num_model_bins_r_trans = self.num_bins_r_trans
+        self.alpha2 = float(opts["--z-evol-obj"])
+        self.redshift_evolution_in_distortion_matrix = self.redshift_evolution
+
+
 def reduce_rows(rows):
     """The parent-process reduction of py/picca/bin/picca_cf.py:466-484."""
     rows = np.array(rows)

@@ -1,10 +1,12 @@
-"""Pins the distortion-matrix oracle against picca's golden dmat files.
+"""Pins the distortion-matrix oracles against picca's golden dmat and xdmat files.
 
 Replays test_dmat / test_dmat_cross / test_dmat_zcuts / test_dmat_cross_zcuts
 (py/picca/tests/test_3_cor.py:318-349, :490-524, :858-893, :931-968) with the oracle:
 brute-force neighbours, the legacy numpy stream seeded with the first healpix
 (py/picca/bin/picca_dmat.py:36), the C restatement of compute_dmat_forest_pairs_fast and
-the merge/normalisation of py/picca/bin/picca_dmat.py:493-519.
+the merge/normalisation of py/picca/bin/picca_dmat.py:493-519.  The same for the
+cross-correlation: test_xdmat / test_xdmat_zcuts (py/picca/tests/test_3_cor.py:666-698,
+:1004-1039) through the C restatement of xcf's compute_dmat_forest_pairs_fast.
 """
 import numpy as np
 import pytest
@@ -45,6 +47,31 @@ def test_oracle_reproduces_golden_dmat(case):
     setup = cor_cases.Setup(cor_cases.DMAT_CASES[case], dmat=True)
     out, stats = oracle_dmat(setup)
     cols, npall, npused = dmat_columns(out, setup.redshift_evolution_in_distortion_matrix)
+    golden = cor_cases.FITS(cor_cases.golden_path(case))
+    header = golden["DMAT"].read_header()
+    assert npall == header["NPALL"]
+    assert npused == header["NPUSED"]
+    cor_cases.compare_to_golden(cols, cor_cases.golden_columns(case))
+
+
+def oracle_xdmat(setup):
+    params = O.params_from_module(setup)
+    params.alpha2 = setup.alpha2
+    healpixs = sorted(setup.data)
+    neigh = O.xcf_neighbours(setup.data, setup.objs, healpixs, setup.ang_max,
+                             setup.r_par_min, setup.r_par_max, setup.r_trans_max)
+    # picca_xdmat fills the neighbours, then seeds the legacy stream with the first
+    # healpix of the process list (py/picca/bin/picca_xdmat.py:35-37)
+    np.random.seed(healpixs[0])
+    return O.compute_xdmat(params, setup.data, healpixs, neigh, setup.reject)
+
+
+@pytest.mark.parametrize("case", ["xdmat", "xdmat_zcuts"])
+def test_oracle_reproduces_golden_xdmat(case):
+    setup = cor_cases.XdmatSetup(cor_cases.XDMAT_CASES[case])
+    out, stats = oracle_xdmat(setup)
+    assert stats[0] > 0 and stats[1] > 0
+    cols, npall, npused = dmat_columns(out, setup.redshift_evolution, alpha2=setup.alpha2)
     golden = cor_cases.FITS(cor_cases.golden_path(case))
     header = golden["DMAT"].read_header()
     assert npall == header["NPALL"]
