@@ -132,10 +132,12 @@ def main():
     parser.add_argument("--traffic-bytes", type=float, default=-1.0,
                         help="HBM bytes per launch of the pair kernel (default: the "
                              "counter-derived figure in profiles/ for this workload)")
-    parser.add_argument("--mode", default="xi", choices=("xi", "dmat", "continuum", "precision"),
+    parser.add_argument("--mode", default="xi",
+                        choices=("xi", "dmat", "continuum", "precision", "xcf", "xdmat"),
                         help="xi: the pair kernel (default); dmat: the distortion matrix; "
                              "continuum: plain vs learned-seed continuum fits; precision: "
-                             "xi error and edge pairs on the heavy-tailed survey")
+                             "xi error and edge pairs on the heavy-tailed survey; xcf / xdmat: "
+                             "forest x quasar correlation / distortion matrix")
     parser.add_argument("--reject", type=float, default=0.99,
                         help="dmat: fraction of neighbour pairs rejected")
     parser.add_argument("--spectra", type=int, default=200000,
@@ -152,6 +154,8 @@ def main():
         return bench_continuum(args)
     if args.mode == "precision":
         return bench_precision(args)
+    if args.mode in ("xcf", "xdmat"):
+        return bench_xcf(args)
     return bench_xi(args)
 
 
@@ -539,6 +543,242 @@ def bench_precision(args):
                    "overflow_pairs_full": int(full.report["overflow_pairs"])},
     }
     print(json.dumps(result))
+
+
+class XcfGlobals:
+    """picca_xcf / picca_xdmat defaults (py/picca/bin/picca_xcf.py, picca_xdmat.py):
+    r_par in [-200, 200) in 100 bins, r_trans < 200 in 50 bins, model binning = data
+    binning, quasar redshift-evolution exponent 1.44."""
+    num_bins_r_par = 100
+    num_bins_r_trans = 50
+    num_model_bins_r_par = 100
+    num_model_bins_r_trans = 50
+    r_par_min = -200.0
+    r_par_max = 200.0
+    r_trans_max = 200.0
+    z_min_pairs = None
+    z_max_pairs = None
+    zerr_cut_deg = None
+    zerr_cut_kms = None
+    z_ref = 2.25
+    alpha = 2.9
+    alpha2 = 1.44
+    x_correlation = True
+    rmu_binning = False
+    ang_correlation = False
+    remove_same_half_plate_close_pairs = False
+    redshift_evolution_in_distortion_matrix = True
+
+
+def xcf_workload(args):
+    """The 100k-forest survey and its own quasars as the object catalogue (each forest's
+    background quasar at z_qso, weight ((1+z)/(1+z_ref))^(1.44-1)), and the forest x
+    quasar pairs of fill_neighs (py/picca/xcf.py:71-128): angle < ang_max, not the same
+    quasar, r_comov range cut."""
+    import types
+    from picca_amd import synth
+    from picca_amd.constants import Cosmo
+    from picca_amd.neighbours import find_neighbours
+    from picca_amd.utils import compute_ang_max
+
+    survey = synth.make_survey("survey_100k", seed=args.seed, scale=args.scale)
+    store = survey.store
+    cosmo = Cosmo(0.315, verbose=False)
+    g = XcfGlobals
+    qso = types.SimpleNamespace(
+        x_cart=store.x_cart, y_cart=store.y_cart, z_cart=store.z_cart, ra=store.ra,
+        dec=store.dec, cos_dec=store.cos_dec, thingid=store.thingid, z_qso=store.z_qso)
+    qso.r_comov = np.asarray(cosmo.get_r_comov(store.z_qso), dtype=np.float64)
+    qso.dist_m = np.asarray(cosmo.get_dist_m(store.z_qso), dtype=np.float64)
+    qso.weights = ((1 + store.z_qso) / (1 + g.z_ref)) ** (g.alpha2 - 1)
+    z_min2 = float(store.z_qso.min())
+    ang_max = compute_ang_max(cosmo, g.r_trans_max, survey.z_min, z_min2)
+    order = np.argsort(survey.healpix, kind="stable")
+    uniq, counts = np.unique(survey.healpix[order], return_counts=True)
+    row_of_src = np.repeat(np.arange(len(uniq), dtype=np.int32), counts)
+    t = time.time()
+    nl = find_neighbours(store, order, order, ang_max, auto=False, target_store=qso)
+    s_of = nl.source_index()
+    forest = order[s_of]
+    quasar = order[nl.target]
+    ang = nl.angle
+    first = store.r_comov[store.offsets[:-1]]
+    last = store.r_comov[store.offsets[1:] - 1]
+    c = np.cos(ang / 2.0)
+    keep = ((first[forest] - qso.r_comov[quasar]) * c < g.r_par_max) & \
+           ((last[forest] - qso.r_comov[quasar]) * c > g.r_par_min)
+    log("xcf neighbours: %d forest x quasar pairs (%.1f s)" % (int(keep.sum()), time.time() - t))
+    return (survey, store, qso, forest[keep], quasar[keep], ang[keep], row_of_src[s_of][keep],
+            len(uniq), ang_max)
+
+
+def bench_xcf(args):
+    from picca_amd.engine import device_info, get_engine, params_from_globals
+    from picca_amd import _native
+    from picca_amd.xcf import _quasar_forests
+
+    survey, store, qso, forest, quasar, ang, row, num_rows, ang_max = xcf_workload(args)
+    nz = store.nonzero_weight_counts()
+    eng = get_engine(0)
+    info = device_info(0)
+    params = params_from_globals(XcfGlobals)
+    if args.mode == "xcf":
+        evaluated = int(nz[forest].astype(np.int64).sum())
+        eng.set_forests(store, key="bench_xcf")
+        eng.xcf_set_pairs(forest, qso.r_comov[quasar], qso.dist_m[quasar], qso.z_qso[quasar],
+                          qso.weights[quasar], ang, row, num_rows)
+        for w in range(args.warmup):
+            eng.xcf(params)
+        wall, kernel, res = [], [], None
+        for k in range(args.steps):
+            ts = time.perf_counter()
+            res = eng.xcf(params)
+            wall.append(time.perf_counter() - ts)
+            kernel.append(res.report["kernel_ms"])
+            log("xcf step %d: wall %.1f ms, kernel %.1f ms" % (k, wall[-1] * 1e3, kernel[-1]))
+        kernel_ms = float(np.mean(kernel))
+        rep = res.report
+        flops = FLOPS_EVALUATED * evaluated + FLOPS_ACCEPTED_EXTRA * rep["accepted_pairs"]
+        achieved = flops / (kernel_ms / 1e3) / 1e12
+        result = {
+            "metric": "forest x quasar pixel pairs/s (1 GPU)",
+            "value": evaluated / (kernel_ms / 1e3), "unit": "pixel pairs/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": 1e3 * float(np.mean(wall)), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "xcf_100k", "scale": args.scale,
+                       "forests": store.num_forests, "quasars": store.num_forests,
+                       "forest_quasar_pairs": int(len(forest)), "healpix_rows": num_rows,
+                       "evaluated_pixel_pairs": evaluated,
+                       "accepted_pixel_pairs": int(rep["accepted_pairs"]),
+                       "kernel_visited_pixel_pairs": int(rep["visited_pairs"]),
+                       "bins": "100x50", "r_par": "[-200, 200)", "r_trans_max": 200.0,
+                       "ang_max_deg": float(np.degrees(ang_max)), "device": info["name"],
+                       "timing": "value: evaluated pairs over the kernel's HIP-event time; "
+                                 "ms_per_step includes the download of the rows"},
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_VALU_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": achieved / FP64_VALU_PEAK_TFLOPS,
+                         "traffic": None, "flops_per_launch": flops,
+                         "kernel": "picca::xcf_kernel", "kernel_ms": kernel_ms},
+            "cpu_baseline": None,
+        }
+        if not args.no_cpu:
+            from oracle import cf_oracle as O
+            oparams = O.params_from_module(XcfGlobals)
+            rng = np.random.default_rng(1)
+            forests = np.sort(rng.choice(store.num_forests, min(store.num_forests, 1000),
+                                         replace=False))
+            by_forest = np.argsort(forest, kind="stable")
+            starts = np.searchsorted(forest[by_forest], forests)
+            ends = np.searchsorted(forest[by_forest], forests, side="right")
+            nb = XcfGlobals.num_bins_r_par * XcfGlobals.num_bins_r_trans
+            out = [np.zeros(nb) for _ in range(5)] + [np.zeros(nb, dtype=np.int64)]
+            stats = np.zeros(3, dtype=np.int64)
+            sample_eval = 0
+            tc = time.time()
+            for f, s0, s1 in zip(forests, starts, ends):
+                if s1 == s0:
+                    continue
+                idx = by_forest[s0:s1]
+                q = quasar[idx]
+                o0, o1 = store.offsets[f], store.offsets[f + 1]
+                O.xcf_pair(oparams, store.z[o0:o1], store.r_comov[o0:o1], store.dist_m[o0:o1],
+                           store.weights[o0:o1], store.delta[o0:o1], qso.z_qso[q],
+                           qso.r_comov[q], qso.dist_m[q], qso.weights[q], ang[idx], out, stats)
+                sample_eval += int(nz[f]) * len(idx)
+            cpu_s = time.time() - tc
+            result["cpu_baseline"] = {
+                "value": sample_eval / cpu_s, "unit": "pixel pairs/s", "cores": 1,
+                "kind": "port",
+                "sample": "%d random forests (seed 1) with all their quasar neighbours, %.3e "
+                          "pixel pairs, FP64 C oracle (scalar restatement of xcf's "
+                          "compute_xi_forest_pairs_fast)" % (len(forests), sample_eval),
+                "seconds": cpu_s}
+            result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
+            # the GPU on the same sample, as one output row: parity with the FP64 oracle
+            sel = np.isin(forest, forests)
+            eng.xcf_set_pairs(forest[sel], qso.r_comov[quasar[sel]], qso.dist_m[quasar[sel]],
+                              qso.z_qso[quasar[sel]], qso.weights[quasar[sel]], ang[sel],
+                              np.zeros(int(sel.sum()), dtype=np.int32), 1)
+            gres = eng.xcf(params)
+            mask = out[0] > 0
+            xi_g = gres.xi[0][mask] / gres.weights[0][mask]
+            xi_o = out[1][mask] / out[0][mask]
+            rel = np.abs(xi_g - xi_o) / np.maximum(np.abs(xi_o), 1e-300)
+            result["parity_sample"] = {
+                "num_pairs_mismatches": int(np.count_nonzero(gres.num_pairs[0] != out[5])),
+                "accepted_pairs_gpu": int(gres.report["accepted_pairs"]),
+                "accepted_pairs_oracle": int(stats[0]),
+                "edge_pairs_gpu": int(gres.report["edge_pairs"]),
+                "edge_pairs_oracle": int(stats[1]),
+                "xi_max_rel_err": float(rel.max()) if rel.size else 0.0}
+        print(json.dumps(result))
+        return
+
+    # distortion matrix: one seeded legacy-numpy draw per forest x quasar pair
+    keep = np.random.RandomState(args.seed).rand(len(forest)) > args.reject
+    fa, qb, ak = forest[keep], quasar[keep], ang[keep]
+    log("xdmat: %d of %d forest x quasar pairs kept" % (len(fa), len(forest)))
+    dstore, nf = _quasar_forests(store, qso)
+    eng.set_forests(dstore, key="bench_xdmat")
+    flags = np.full(len(fa), _native.PICCA_PAIR_XCF_COUNT, dtype=np.uint8)
+    for w in range(args.warmup):
+        eng.dmat(params, fa, nf + qb, ak, flags)
+    times, out = [], None
+    for k in range(args.steps):
+        t = time.perf_counter()
+        out = eng.dmat(params, fa, nf + qb, ak, flags)
+        times.append(time.perf_counter() - t)
+        log("xdmat step %d: %.2f s" % (k, times[-1]))
+    sec = float(np.mean(times))
+    stats = out[6]
+    result = {
+        "metric": "cross distortion-matrix in-range pixel pairs/s (1 GPU)",
+        "value": stats[0] / sec, "unit": "pixel pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * sec, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "xdmat_100k", "scale": args.scale,
+                   "forests": store.num_forests, "forest_quasar_pairs": int(len(forest)),
+                   "kept_pairs": int(len(fa)), "reject": args.reject,
+                   "in_range_pairs": int(stats[0]), "selected_pairs": int(stats[1]),
+                   "bins": "100x50", "model_bins": "100x50", "device": info["name"],
+                   "timing": "wall time of picca_dmat_run, including the 200 MB download of "
+                             "the matrix"},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        from oracle import cf_oracle as O
+        oparams = O.params_from_module(XcfGlobals)
+        rng = np.random.default_rng(1)
+        forests = np.unique(fa)
+        forests = np.sort(rng.choice(forests, min(len(forests), 40), replace=False))
+        nb = XcfGlobals.num_bins_r_par * XcfGlobals.num_bins_r_trans
+        nm = XcfGlobals.num_model_bins_r_par * XcfGlobals.num_model_bins_r_trans
+        ref = [np.zeros(nb), np.zeros(nb * nm)] + [np.zeros(nm) for _ in range(4)]
+        rstats = np.zeros(4, dtype=np.int64)
+        sel = np.isin(fa, forests)
+        tc = time.time()
+        for f in forests:
+            m = fa == f
+            q = qb[m]
+            o0, o1 = store.offsets[f], store.offsets[f + 1]
+            O.xdmat_forest(oparams, store.log_lambda[o0:o1], store.r_comov[o0:o1],
+                           store.dist_m[o0:o1], store.z[o0:o1], store.weights[o0:o1],
+                           qso.r_comov[q], qso.dist_m[q], qso.z_qso[q], qso.weights[q], ak[m],
+                           ref, int(store.order[f]), rstats)
+        cpu_s = time.time() - tc
+        mine = eng.dmat(params, fa[sel], nf + qb[sel], ak[sel], flags[sel])
+        err = float(np.abs(mine[1].ravel() - ref[1]).max() / np.abs(ref[1]).max())
+        result["cpu_baseline"] = {
+            "value": rstats[0] / cpu_s, "unit": "pixel pairs/s", "cores": 1, "kind": "port",
+            "sample": "%d random forests (seed 1) with their kept quasars, %d in-range pixel "
+                      "pairs, FP64 C oracle (scalar restatement of xcf's "
+                      "compute_dmat_forest_pairs_fast)" % (len(forests), int(rstats[0])),
+            "seconds": cpu_s}
+        result["parity_sample"] = {"dmat_max_abs_err_over_max": err,
+                                   "counts_equal": bool(list(mine[6][:2]) == list(rstats[:2]))}
+    print(json.dumps(result))
+
 
 if __name__ == "__main__":
     main()

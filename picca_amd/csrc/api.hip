@@ -195,6 +195,19 @@ int picca_set_forests(picca_ctx *ctx, int64_t num_forests, const int64_t *offset
     ctx->have_result = false;
     ctx->num_pairs = 0;
     ctx->h_offsets.assign(offsets, offsets + num_forests + 1);
+    ctx->h_dmin.assign((size_t)num_forests, 0.0);
+    ctx->h_dmax.assign((size_t)num_forests, 0.0);
+    parallel_for(num_forests, [&](int64_t lo, int64_t hi) {
+        for (int64_t k = lo; k < hi; ++k) {
+            double a = INFINITY, b = -INFINITY;
+            for (int64_t q = offsets[k]; q < offsets[k + 1]; ++q) {
+                a = std::min(a, dist_m[q]);
+                b = std::max(b, dist_m[q]);
+            }
+            ctx->h_dmin[k] = a;
+            ctx->h_dmax[k] = b;
+        }
+    });
     PICCA_HIP_CHECK(ctx->off.upload(offsets, (size_t)num_forests + 1, ctx->stream));
     // the pair kernel reads pixel batches of four, up to three past a forest's end
     const size_t pad = picca::PIXEL_PAD;
@@ -482,6 +495,8 @@ int picca_xcf_set_pairs(picca_ctx *ctx, int64_t num_pairs, const int32_t *pair_f
     // of neighbouring r), which only reorders the additions of the sums
     std::vector<picca::XcfPairRecord> rec((size_t)std::max<int64_t>(num_pairs, 1));
     const int64_t ngroups = (int64_t)forest.size();
+    ctx->hx_rt_lo.assign((size_t)num_pairs, 0.0);
+    ctx->hx_rt_hi.assign((size_t)num_pairs, 0.0);
     parallel_for(ngroups, [&](int64_t lo, int64_t hi) {
         std::vector<int64_t> idx;
         for (int64_t g = lo; g < hi; ++g) {
@@ -505,6 +520,9 @@ int picca_xcf_set_pairs(picca_ctx *ctx, int64_t num_pairs, const int32_t *pair_f
                     r.c = std::cos(pair_ang[s] / 2);
                     r.s = std::sin(pair_ang[s] / 2);
                 }
+                const int32_t f = forest[g];
+                ctx->hx_rt_lo[k] = (ctx->h_dmin[f] + r.d) * r.s;
+                ctx->hx_rt_hi[k] = (ctx->h_dmax[f] + r.d) * r.s;
             }
         }
     });
@@ -519,6 +537,61 @@ int picca_xcf_set_pairs(picca_ctx *ctx, int64_t num_pairs, const int32_t *pair_f
     ctx->xcf_pairs = num_pairs;
     ctx->xcf_rows = num_rows;
     ctx->xcf_groups = ngroups;
+    ctx->hx_begin = std::move(begin);
+    ctx->hx_forest = std::move(forest);
+    ctx->hx_row = std::move(row);
+    ctx->x_pairs_version += 1;
+    return PICCA_OK;
+}
+
+// Row windows of the cross-correlation's fast loop: windows of `rows` r_trans rows, each
+// with the pairs whose r_trans range (one row of margin on each side) reaches it.  A pair
+// whose r_trans can round up to r_trans_max also goes to the first window: the reference
+// bins it in bin 0 of the next r_par row.
+static int xcf_plan(picca_ctx *ctx, const picca::XiConst &C, int rows) {
+    if (ctx->x_plan_version == ctx->x_pairs_version && ctx->x_plan_nt == C.nt &&
+        ctx->x_plan_rows == rows && ctx->x_plan_rtmax == C.rtmax)
+        return PICCA_OK;
+    const int nwin = (C.nt + rows - 1) / rows;
+    ctx->x_windows.clear();
+    for (int w = 0; w < nwin; ++w) ctx->x_windows.push_back(std::make_unique<picca_ctx::XcfWindow>());
+    const int64_t ngroups = ctx->xcf_groups;
+    const double inv_w = C.nt_d / C.rtmax;
+    for (int w = 0; w < nwin; ++w) {
+        picca_ctx::XcfWindow &win = *ctx->x_windows[(size_t)w];
+        win.t0 = w * rows;
+        win.rows = std::min(rows, C.nt - win.t0);
+        const bool last = w == nwin - 1;
+        std::vector<int64_t> idx, begin;
+        std::vector<int32_t> forest, row;
+        for (int64_t g = 0; g < ngroups; ++g) {
+            const int64_t b = ctx->hx_begin[g], e = ctx->hx_begin[g + 1];
+            const int64_t before = (int64_t)idx.size();
+            for (int64_t k = b; k < e; ++k) {
+                const double lo = ctx->hx_rt_lo[k] * inv_w * (1 - 1e-9) - 1.0;
+                const double hi = ctx->hx_rt_hi[k] * inv_w * (1 + 1e-9) + 1.0;
+                bool take = hi >= win.t0 && (last || lo < win.t0 + win.rows);
+                if (w == 0 && hi >= C.nt_d) take = true;
+                if (take) idx.push_back(k);
+            }
+            if ((int64_t)idx.size() > before) {
+                begin.push_back(before);
+                forest.push_back(ctx->hx_forest[g]);
+                row.push_back(ctx->hx_row[g]);
+            }
+        }
+        begin.push_back((int64_t)idx.size());
+        win.groups = (int64_t)forest.size();
+        PICCA_HIP_CHECK(win.idx.upload(idx.data(), idx.size(), ctx->stream));
+        PICCA_HIP_CHECK(win.begin.upload(begin.data(), begin.size(), ctx->stream));
+        PICCA_HIP_CHECK(win.forest.upload(forest.data(), forest.size(), ctx->stream));
+        PICCA_HIP_CHECK(win.row.upload(row.data(), row.size(), ctx->stream));
+        PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+    ctx->x_plan_version = ctx->x_pairs_version;
+    ctx->x_plan_nt = C.nt;
+    ctx->x_plan_rows = rows;
+    ctx->x_plan_rtmax = C.rtmax;
     return PICCA_OK;
 }
 
@@ -528,11 +601,41 @@ int picca_xcf_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_re
     int mode = 0;
     int status = build_xi_const(params, C, mode);
     if (status != PICCA_OK) return status;
+    // the fast loop classifies pairs from (r1 - r2) (cos np / span) - r_par_min np / span and
+    // (d1 + d2) (sin nt / r_trans_max): widen the edge margins by a bound of their rounding
+    {
+        const double u = std::ldexp(1.0, -53);
+        const double err_p = 16.0 * u * ctx->max_abs_r * C.inv_p +
+                             8.0 * u * (C.np_d + std::fabs(C.rpmin) * C.inv_p + 2.0);
+        const double err_t = 16.0 * u * ctx->max_abs_d * C.inv_t + 8.0 * u * (C.nt_d + 2.0);
+        C.fast_p += err_p;
+        C.fast_t += err_t;
+        C.half_p = 0.5 - C.fast_p;
+        C.half_t = 0.5 - C.fast_t;
+        if (!(C.half_p > 0.25 && C.half_t > 0.25))
+            return fail(PICCA_E_VALUE, "edge_tol is too large compared with the bin width");
+    }
     const size_t static_lds = 1024;
+    const bool fast_mode = mode == picca::MODE_STANDARD && !params->has_z_min_pairs &&
+                           !params->has_z_max_pairs;
+    // rows of the padded window histogram that fit in LDS
+    const int64_t per_row = (int64_t)(C.np + 2) * (int64_t)picca::XI_HIST_BIN_BYTES;
+    const int64_t fit_rows = (ctx->lds_limit - (int64_t)static_lds) / per_row - 2;
+    const bool fast = fast_mode && fit_rows >= 1;
     const size_t hist = (size_t)C.nb * picca::XI_HIST_BIN_BYTES;
-    const bool lds_hist = (int64_t)(hist + static_lds) <= ctx->lds_limit;
-    const size_t lds = lds_hist ? hist : 16;
+    const bool lds_hist = fast || (int64_t)(hist + static_lds) <= ctx->lds_limit;
+    int rows = C.nt;
+    if (fast) {
+        rows = (int)std::min<int64_t>(C.nt, fit_rows);
+        // balance the windows: the same number of rows in each
+        const int nwin = (C.nt + rows - 1) / rows;
+        rows = (C.nt + nwin - 1) / nwin;
+    }
     PICCA_HIP_CHECK(hipSetDevice(ctx->device));
+    if (fast && rows < C.nt) {
+        status = xcf_plan(ctx, C, rows);
+        if (status != PICCA_OK) return status;
+    }
     const int64_t nrows = ctx->xcf_rows;
     const int64_t nout = nrows * C.nb;
     PICCA_HIP_CHECK(ctx->o_bins.reserve((size_t)std::max<int64_t>(nout, 1)));
@@ -540,17 +643,45 @@ int picca_xcf_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_re
     PICCA_HIP_CHECK(ctx->stats.reserve(picca::XI_NSTATS));
     PICCA_HIP_CHECK(picca::launch_zero_u64(reinterpret_cast<unsigned long long *>(ctx->o_bins.ptr),
                                            nout * (int64_t)(sizeof(picca::XiBin) / 8), ctx->stream));
-    PICCA_HIP_CHECK(picca::launch_zero_u64(ctx->queue.ptr, 1, ctx->stream));
     PICCA_HIP_CHECK(picca::launch_zero_u64(ctx->stats.ptr, picca::XI_NSTATS, ctx->stream));
     picca::ForestView F{ctx->pix.ptr, ctx->z.ptr, ctx->r.ptr, ctx->d.ptr, ctx->w.ptr, ctx->dw.ptr,
                         ctx->ll.ptr, ctx->off.ptr, ctx->zq.ptr, ctx->order.ptr, ctx->mono.ptr};
-    picca::XcfView X{ctx->x_rec.ptr, ctx->x_forest.ptr, ctx->x_begin.ptr, ctx->x_row.ptr,
-                     ctx->xcf_groups};
     picca::XiOut O{ctx->o_bins.ptr, ctx->queue.ptr, ctx->stats.ptr};
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->num_cus, ctx->xcf_groups));
     PICCA_HIP_CHECK(hipEventRecord(ctx->ev_start, ctx->stream));
-    if (ctx->xcf_groups > 0)
-        PICCA_HIP_CHECK(picca::launch_xcf(mode, lds_hist, C, F, X, O, grid, lds, ctx->stream));
+    auto run = [&](const picca::XcfView &X, const picca::XiConst &K) -> int {
+        if (X.count <= 0) return PICCA_OK;
+        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->num_cus, X.count));
+        const size_t lds = fast ? (size_t)(K.np + 2) * (size_t)(K.win_rows + 2) *
+                                      picca::XI_HIST_BIN_BYTES
+                                : (lds_hist ? hist : 16);
+        PICCA_HIP_CHECK(picca::launch_zero_u64(ctx->queue.ptr, 1, ctx->stream));
+        PICCA_HIP_CHECK(picca::launch_xcf(mode, lds_hist, fast, K, F, X, O, grid, lds,
+                                          ctx->stream));
+        return PICCA_OK;
+    };
+    if (fast && rows < C.nt) {
+        const int nwin = (int)ctx->x_windows.size();
+        for (int w = 0; w < nwin; ++w) {
+            const picca_ctx::XcfWindow &win = *ctx->x_windows[(size_t)w];
+            picca::XiConst K = C;
+            K.win_t0 = win.t0;
+            K.win_rows = win.rows;
+            K.win_last = w == nwin - 1;
+            picca::XcfView X{ctx->x_rec.ptr, win.idx.ptr, win.forest.ptr, win.begin.ptr,
+                             win.row.ptr, win.groups};
+            status = run(X, K);
+            if (status != PICCA_OK) return status;
+        }
+    } else {
+        picca::XiConst K = C;
+        K.win_t0 = 0;
+        K.win_rows = C.nt;
+        K.win_last = 1;
+        picca::XcfView X{ctx->x_rec.ptr, nullptr, ctx->x_forest.ptr, ctx->x_begin.ptr,
+                         ctx->x_row.ptr, ctx->xcf_groups};
+        status = run(X, K);
+        if (status != PICCA_OK) return status;
+    }
     PICCA_HIP_CHECK(hipEventRecord(ctx->ev_stop, ctx->stream));
     unsigned long long stats[picca::XI_NSTATS];
     PICCA_HIP_CHECK(hipMemcpyAsync(stats, ctx->stats.ptr, sizeof(stats), hipMemcpyDeviceToHost,

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <string>
 #include <thread>
+#include <memory>
 #include <vector>
 
 #include "../../include/picca_hip.h"
@@ -111,6 +112,7 @@ struct picca_ctx {
     bool has_log_lambda = false;
     bool has_order = false;
     std::vector<int64_t> h_offsets;
+    std::vector<double> h_dmin, h_dmax;   // per forest: smallest / largest dist_m
     DeviceBuffer<double> z, r, d, w, dw, ll, zq;
     DeviceBuffer<picca::PixelRecord> pix;
     DeviceBuffer<int64_t> off;
@@ -171,6 +173,23 @@ struct picca_ctx {
     DeviceBuffer<int32_t> x_forest, x_row;
     DeviceBuffer<int64_t> x_begin;
     int64_t xcf_groups = 0;
+    // host copies of the groups and, per pair in record order, the smallest and largest
+    // r_trans any pixel of its forest can have with the quasar (for the row windows)
+    std::vector<int64_t> hx_begin;
+    std::vector<int32_t> hx_forest, hx_row;
+    std::vector<double> hx_rt_lo, hx_rt_hi;
+    // row windows of the last plan: per window, the pair indices and groups on the device
+    struct XcfWindow {
+        int32_t t0 = 0, rows = 0;
+        int64_t groups = 0;
+        DeviceBuffer<int64_t> idx, begin;
+        DeviceBuffer<int32_t> forest, row;
+    };
+    std::vector<std::unique_ptr<XcfWindow>> x_windows;
+    int64_t x_plan_version = -1;
+    int32_t x_plan_nt = 0, x_plan_rows = 0;
+    double x_plan_rtmax = 0;
+    int64_t x_pairs_version = 0;
 
     // xi results
     int64_t out_nb = 0;

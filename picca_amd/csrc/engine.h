@@ -53,6 +53,9 @@ struct XiConst {
     double zerr_kms;
     double np_d, nt_d, nb_d;
     double ntp_d;           // nt + 1
+    // r_trans row window of the cross-correlation kernel's LDS histogram: rows
+    // [win_t0, win_t0 + win_rows) of the output, padded to (np + 2) x (win_rows + 2) bins
+    int32_t win_t0, win_rows, win_last, win_pad_;
 };
 
 // One pixel as the pair kernel streams it through the scalar cache.
@@ -289,15 +292,17 @@ struct XcfPairRecord {
 
 struct XcfView {
     const XcfPairRecord *__restrict__ rec;  // [num_pairs], grouped by forest
+    const int64_t *__restrict__ idx;        // group-local pair k -> rec[idx[begin + k]]
+                                            // (nullptr: rec[begin + k])
     const int32_t *__restrict__ forest;     // [count] forest of each group
     const int64_t *__restrict__ begin;      // [count + 1] pair range of each group
     const int32_t *__restrict__ row;        // [count] output row of each group
     int64_t count;
 };
 
-hipError_t launch_xcf(int mode, bool lds_hist, const XiConst &C, const ForestView &F,
-                      const XcfView &X, const XiOut &O, int grid, size_t lds,
-                      hipStream_t stream);
+hipError_t launch_xcf(int mode, bool lds_hist, bool fast, const XiConst &C,
+                      const ForestView &F, const XcfView &X, const XiOut &O, int grid,
+                      size_t lds, hipStream_t stream);
 
 hipError_t launch_xi(int mode, bool xcorr, bool general, bool lds_hist, const XiConst &C,
                      const ForestView &F, const PairView &P, const GroupView &G,

@@ -135,7 +135,7 @@ def dmat_params():
 def _quasar_forests(store, obj):
     """The deltas' store followed by one one-pixel forest per quasar (order -1)."""
     nf = len(store.offsets) - 1
-    nq = len(obj.objects)
+    nq = len(obj.z_qso)
     offsets = np.concatenate([store.offsets,
                               store.offsets[-1] + np.arange(1, nq + 1, dtype=np.int64)])
     ll = store.log_lambda if store.log_lambda is not None else np.zeros(len(store.z))

@@ -54,15 +54,17 @@ def test_xcf_rows_match_oracle(case):
     assert report["overflow_pairs"] == stats[2]
 
 
-@pytest.mark.parametrize("mode", ["rmu", "narrow"])
+@pytest.mark.parametrize("mode", ["rmu", "narrow", "windows"])
 def test_xcf_modes_match_oracle(mode):
     setup = cor_cases.XcfSetup(cor_cases.XCF_CASES["xcf"])
     if mode == "rmu":
         setup.rmu_binning = True
         setup.r_par_min, setup.r_par_max = -1.0, 1.0
-    else:   # asymmetric r_par range with the r_comov neighbour cut active
+    elif mode == "narrow":   # asymmetric r_par range with the r_comov neighbour cut active
         setup.r_par_min, setup.r_par_max = -20.0, 45.0
         setup.num_bins_r_par = 13
+    else:   # 100 x 50 bins: the LDS histogram is split into windows of r_trans rows
+        setup.num_bins_r_par, setup.num_bins_r_trans = 100, 50
     set_xcf_globals(setup)
     healpixs = sorted(setup.data)
     xcf.fill_neighs(healpixs)
@@ -76,6 +78,10 @@ def test_xcf_modes_match_oracle(mode):
     assert np.array_equal(mine[5], ref[5])
     for k in range(5):
         np.testing.assert_allclose(mine[k], ref[k], rtol=RTOL, atol=1e-12)
+    report = xcf.last_report
+    assert report["accepted_pairs"] == st[0]
+    assert report["edge_pairs"] == st[1]
+    assert report["overflow_pairs"] == st[2]
 
 
 @pytest.mark.parametrize("case", sorted(cor_cases.XCF_CASES))
@@ -228,3 +234,70 @@ def test_xdmat_driver_reproduces_golden(case, tmp_path):
         for col in hdu.get_colnames():
             result[col] = hdu[col][:]
     cor_cases.compare_to_golden(result, cor_cases.golden_columns(case))
+
+
+def test_xcf_many_rows_match_oracle():
+    """A synthetic survey crossed with its own quasars at 100 x 50 bins: many healpix rows
+    per workgroup, so each workgroup moves its LDS histogram to HBM many times; every row
+    must equal the oracle's (a wave adding to a bin while the histogram is being moved
+    would lose pairs)."""
+    import types
+    from picca_amd import synth
+    from picca_amd.constants import Cosmo
+    from picca_amd.engine import get_engine, params_from_globals
+    from picca_amd.neighbours import find_neighbours
+
+    s = synth.make_survey("survey_small", seed=4, scale=1.0)
+    store = s.store
+    cosmo = Cosmo(0.315, verbose=False)
+
+    class G:
+        num_bins_r_par, num_bins_r_trans = 100, 50
+        num_model_bins_r_par = num_model_bins_r_trans = None
+        r_par_min, r_par_max, r_trans_max = -100.0, 100.0, 100.0
+        z_min_pairs = z_max_pairs = zerr_cut_deg = zerr_cut_kms = None
+        z_ref, alpha, alpha2 = 2.25, 2.9, 1.44
+        x_correlation, rmu_binning, ang_correlation = True, False, False
+        remove_same_half_plate_close_pairs = False
+
+    qso = types.SimpleNamespace(x_cart=store.x_cart, y_cart=store.y_cart, z_cart=store.z_cart,
+                                ra=store.ra, dec=store.dec, cos_dec=store.cos_dec,
+                                thingid=store.thingid, z_qso=store.z_qso)
+    qso.r_comov = np.asarray(cosmo.get_r_comov(store.z_qso), dtype=np.float64)
+    qso.dist_m = np.asarray(cosmo.get_dist_m(store.z_qso), dtype=np.float64)
+    qso.weights = ((1 + store.z_qso) / 3.25) ** 0.44
+    ang_max = 2 * np.arcsin(G.r_trans_max / (2 * float(cosmo.get_dist_m(s.z_min))))
+    # rows: healpix pixels at nside 256 (a few forests each), so that the workgroups
+    # change rows often
+    from picca_amd.healpix import ang2pix
+    hp = ang2pix(256, np.pi / 2 - store.dec, store.ra)
+    order = np.argsort(hp, kind="stable")
+    uniq, counts = np.unique(hp[order], return_counts=True)
+    row_of_src = np.repeat(np.arange(len(uniq), dtype=np.int32), counts)
+    nl = find_neighbours(store, order, order, ang_max, auto=False, target_store=qso)
+    src = nl.source_index()
+    forest, quasar, ang, row = order[src], order[nl.target], nl.angle, row_of_src[src]
+    assert len(uniq) > 100
+    eng = get_engine(0)
+    eng.set_forests(store)
+    eng.xcf_set_pairs(forest, qso.r_comov[quasar], qso.dist_m[quasar], qso.z_qso[quasar],
+                      qso.weights[quasar], ang, row, len(uniq))
+    res = eng.xcf(params_from_globals(G))
+    params = O.params_from_module(G)
+    nb = G.num_bins_r_par * G.num_bins_r_trans
+    total = 0
+    for r in range(len(uniq)):
+        out = [np.zeros(nb) for _ in range(5)] + [np.zeros(nb, dtype=np.int64)]
+        stats = np.zeros(3, dtype=np.int64)
+        for f in np.unique(forest[row == r]):
+            idx = np.nonzero(forest == f)[0]
+            q = quasar[idx]
+            o0, o1 = store.offsets[f], store.offsets[f + 1]
+            O.xcf_pair(params, store.z[o0:o1], store.r_comov[o0:o1], store.dist_m[o0:o1],
+                       store.weights[o0:o1], store.delta[o0:o1], qso.z_qso[q], qso.r_comov[q],
+                       qso.dist_m[q], qso.weights[q], ang[idx], out, stats)
+        assert np.array_equal(res.num_pairs[r], out[5]), r
+        np.testing.assert_allclose(res.weights[r], out[0], rtol=RTOL, atol=1e-12)
+        np.testing.assert_allclose(res.xi[r], out[1], rtol=RTOL, atol=1e-9)
+        total += int(out[5].sum())
+    assert total > 0 and res.report["accepted_pairs"] == total
