@@ -80,10 +80,11 @@ def cpu_cores():
         return os.cpu_count() or 1
 
 
-def measured_traffic(workload, scale):
-    """HBM bytes per pair-kernel launch measured with FETCH_SIZE + WRITE_SIZE (each scaled
-    by the same counters on a plain copy), as recorded beside the profiles."""
-    path = os.path.join(ROOT, "profiles", "xi_kernel_hbm_traffic.json")
+def measured_traffic(workload, scale, kernel="xi_kernel"):
+    """HBM bytes of one step of the pair kernel (all its launches), measured with FETCH_SIZE +
+    WRITE_SIZE (each scaled by the same counters on a plain copy), as recorded beside the
+    profiles."""
+    path = os.path.join(ROOT, "profiles", "%s_hbm_traffic.json" % kernel)
     try:
         with open(path) as f:
             table = json.load(f)
@@ -627,6 +628,10 @@ def bench_xcf(args):
         eng.set_forests(store, key="bench_xcf")
         eng.xcf_set_pairs(forest, qso.r_comov[quasar], qso.dist_m[quasar], qso.z_qso[quasar],
                           qso.weights[quasar], ang, row, num_rows)
+        if args.traffic_bytes < 0:
+            args.traffic_bytes = measured_traffic("xcf_100k", args.scale, "xcf_kernel")
+        if args.copy_probe > 0:
+            copy_probe(eng, args.copy_probe)
         for w in range(args.warmup):
             eng.xcf(params)
         wall, kernel, res = [], [], None
@@ -658,7 +663,12 @@ def bench_xcf(args):
                                  "ms_per_step includes the download of the rows"},
             "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_VALU_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": achieved / FP64_VALU_PEAK_TFLOPS,
-                         "traffic": None, "flops_per_launch": flops,
+                         "traffic": (args.traffic_bytes / rep["launches"]
+                                     if args.traffic_bytes > 0 else None),
+                         "flops_per_launch": flops / rep["launches"],
+                         "launches_per_step": int(rep["launches"]),
+                         "launch_avg_ms": kernel_ms / rep["launches"],
+                         "traffic_per_step": args.traffic_bytes or None,
                          "kernel": "picca::xcf_kernel", "kernel_ms": kernel_ms},
             "cpu_baseline": None,
         }

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PICCA_HIP_ABI_VERSION 1
+#define PICCA_HIP_ABI_VERSION 2
 
 #define PICCA_OK 0
 #define PICCA_E_VALUE (-1)   /* invalid argument: picca raises ValueError          */
@@ -86,6 +86,7 @@ typedef struct picca_xi_report {
     int64_t tasks;           /* (pair, 64-pixel chunk of delta1) work items         */
     int64_t empty_tasks;     /* work items whose pruned pixel window was empty      */
     double kernel_ms;        /* device time of the pair kernel (HIP events)         */
+    int64_t launches;        /* pair-kernel launches inside kernel_ms               */
 } picca_xi_report;
 
 typedef struct picca_ctx picca_ctx;

@@ -64,6 +64,7 @@ class XiReport(ctypes.Structure):
         ("tasks", ctypes.c_int64),
         ("empty_tasks", ctypes.c_int64),
         ("kernel_ms", ctypes.c_double),
+        ("launches", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -112,6 +113,10 @@ _lib = None
 _load_error = None
 
 
+# PICCA_HIP_ABI_VERSION of include/picca_hip.h that these structures mirror
+ABI_VERSION = 2
+
+
 def exported_symbols():
     """Names of the C-ABI functions this binding expects the library to export."""
     return sorted(_PROTOTYPES) + ["picca_last_error"]
@@ -137,6 +142,9 @@ def load():
         fn.restype = ctypes.c_int
     lib.picca_last_error.argtypes = []
     lib.picca_last_error.restype = ctypes.c_char_p
+    if lib.picca_abi_version() != ABI_VERSION:
+        raise RuntimeError("%s has ABI version %d, these bindings expect %d: rebuild it"
+                           % (LIBRARY_PATH, lib.picca_abi_version(), ABI_VERSION))
     _lib = lib
     return lib
 

@@ -261,6 +261,10 @@ int picca_set_forests(picca_ctx *ctx, int64_t num_forests, const int64_t *offset
     }
     ctx->has_order = order != nullptr;
     if (order) PICCA_HIP_CHECK(ctx->order.upload(order, (size_t)num_forests, ctx->stream));
+    if (order)
+        ctx->h_order.assign(order, order + num_forests);
+    else
+        ctx->h_order.clear();
     PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     ctx->num_forests = num_forests;
     ctx->num_pixels = npix;
@@ -428,6 +432,7 @@ int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_rep
         report->tasks = (int64_t)stats[picca::XI_STAT_TASKS];
         report->empty_tasks = (int64_t)stats[picca::XI_STAT_EMPTY_TASKS];
         report->kernel_ms = (double)ms;
+        report->launches = ngroups > 0 ? 1 : 0;
     }
     return PICCA_OK;
 }
@@ -453,6 +458,42 @@ int picca_xi_download(picca_ctx *ctx, double *weights, double *xi, double *r_par
         if (num_pairs) num_pairs[k] = (int64_t)b.nb;
     }
     return PICCA_OK;
+}
+
+// Claim order of the cross-correlation's groups: runs of consecutive groups of one output
+// row, each holding about half of the work still unclaimed over the grid (guided
+// schedule, work = pixels of the forest x quasars of the group).  A workgroup moves its
+// LDS histogram to HBM when its row changes, so whole-row claims early in the launch
+// replace one flush per group by one per row, and the small claims at the end keep the
+// grid balanced.  Returns the first group of each claim and, last, the group count.
+static std::vector<int64_t> xcf_chunks(const std::vector<int64_t> &begin,
+                                       const std::vector<int32_t> &forest,
+                                       const std::vector<int32_t> &row,
+                                       const std::vector<int64_t> &offsets, int grid) {
+    const int64_t ngroups = (int64_t)forest.size();
+    std::vector<double> work((size_t)ngroups);
+    double total = 0;
+    for (int64_t g = 0; g < ngroups; ++g) {
+        const int32_t f = forest[(size_t)g];
+        work[(size_t)g] = (double)(offsets[(size_t)f + 1] - offsets[(size_t)f]) *
+                          (double)(begin[(size_t)g + 1] - begin[(size_t)g]);
+        total += work[(size_t)g];
+    }
+    std::vector<int64_t> chunk;
+    double left = total;
+    const double share = 1.0 / (2.0 * std::max(grid, 1));
+    for (int64_t g = 0; g < ngroups;) {
+        chunk.push_back(g);
+        const double target = left * share;
+        double w = work[(size_t)g];
+        int64_t e = g + 1;
+        while (e < ngroups && row[(size_t)e] == row[(size_t)g] && w + work[(size_t)e] <= target)
+            w += work[(size_t)e++];
+        left -= w;
+        g = e;
+    }
+    chunk.push_back(ngroups);
+    return chunk;
 }
 
 int picca_xcf_set_pairs(picca_ctx *ctx, int64_t num_pairs, const int32_t *pair_forest,
@@ -533,7 +574,10 @@ int picca_xcf_set_pairs(picca_ctx *ctx, int64_t num_pairs, const int32_t *pair_f
     PICCA_HIP_CHECK(ctx->x_forest.upload(forest.data(), forest.size(), ctx->stream));
     PICCA_HIP_CHECK(ctx->x_row.upload(row.data(), row.size(), ctx->stream));
     PICCA_HIP_CHECK(ctx->x_begin.upload(begin.data(), begin.size(), ctx->stream));
+    const std::vector<int64_t> chunk = xcf_chunks(begin, forest, row, ctx->h_offsets, ctx->num_cus);
+    PICCA_HIP_CHECK(ctx->x_chunk.upload(chunk.data(), chunk.size(), ctx->stream));
     PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->xcf_chunks = (int64_t)chunk.size() - 1;
     ctx->xcf_pairs = num_pairs;
     ctx->xcf_rows = num_rows;
     ctx->xcf_groups = ngroups;
@@ -582,6 +626,10 @@ static int xcf_plan(picca_ctx *ctx, const picca::XiConst &C, int rows) {
         }
         begin.push_back((int64_t)idx.size());
         win.groups = (int64_t)forest.size();
+        const std::vector<int64_t> chunk = xcf_chunks(begin, forest, row, ctx->h_offsets,
+                                                      ctx->num_cus);
+        win.chunks = (int64_t)chunk.size() - 1;
+        PICCA_HIP_CHECK(win.chunk.upload(chunk.data(), chunk.size(), ctx->stream));
         PICCA_HIP_CHECK(win.idx.upload(idx.data(), idx.size(), ctx->stream));
         PICCA_HIP_CHECK(win.begin.upload(begin.data(), begin.size(), ctx->stream));
         PICCA_HIP_CHECK(win.forest.upload(forest.data(), forest.size(), ctx->stream));
@@ -648,8 +696,10 @@ int picca_xcf_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_re
                         ctx->ll.ptr, ctx->off.ptr, ctx->zq.ptr, ctx->order.ptr, ctx->mono.ptr};
     picca::XiOut O{ctx->o_bins.ptr, ctx->queue.ptr, ctx->stats.ptr};
     PICCA_HIP_CHECK(hipEventRecord(ctx->ev_start, ctx->stream));
+    int64_t launches = 0;
     auto run = [&](const picca::XcfView &X, const picca::XiConst &K) -> int {
         if (X.count <= 0) return PICCA_OK;
+        ++launches;
         const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->num_cus, X.count));
         const size_t lds = fast ? (size_t)(K.np + 2) * (size_t)(K.win_rows + 2) *
                                       picca::XI_HIST_BIN_BYTES
@@ -668,7 +718,7 @@ int picca_xcf_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_re
             K.win_rows = win.rows;
             K.win_last = w == nwin - 1;
             picca::XcfView X{ctx->x_rec.ptr, win.idx.ptr, win.forest.ptr, win.begin.ptr,
-                             win.row.ptr, win.groups};
+                             win.row.ptr, win.chunk.ptr, win.groups, win.chunks};
             status = run(X, K);
             if (status != PICCA_OK) return status;
         }
@@ -678,7 +728,7 @@ int picca_xcf_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_re
         K.win_rows = C.nt;
         K.win_last = 1;
         picca::XcfView X{ctx->x_rec.ptr, nullptr, ctx->x_forest.ptr, ctx->x_begin.ptr,
-                         ctx->x_row.ptr, ctx->xcf_groups};
+                         ctx->x_row.ptr, ctx->x_chunk.ptr, ctx->xcf_groups, ctx->xcf_chunks};
         status = run(X, K);
         if (status != PICCA_OK) return status;
     }
@@ -701,6 +751,7 @@ int picca_xcf_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_re
         report->tasks = (int64_t)stats[picca::XI_STAT_TASKS];
         report->empty_tasks = (int64_t)stats[picca::XI_STAT_EMPTY_TASKS];
         report->kernel_ms = (double)ms;
+        report->launches = launches;
     }
     return PICCA_OK;
 }
@@ -893,7 +944,10 @@ int picca_dmat_run(picca_ctx *ctx, const picca_cf_params *params, int64_t num_pa
                           ctx->dstats.ptr};
         PICCA_HIP_CHECK(picca::launch_dmat_pass(C, F, P, W, O, pass_lds, ctx->stream));
         const int groups = (int)std::max<int64_t>(1, std::min<int64_t>(32, count / 16));
-        PICCA_HIP_CHECK(picca::launch_dmat_fill(C, F, P, W, O, groups, ctx->stream));
+        bool quasar = true;   // every delta2 of the batch unprojected
+        for (int64_t k = 0; k < count && quasar; ++k)
+            quasar = ctx->h_order[(size_t)pair_b[p0 + k]] < 0;
+        PICCA_HIP_CHECK(picca::launch_dmat_fill(C, F, P, W, O, groups, quasar, ctx->stream));
     }
 
     std::vector<double> h_wdm((size_t)nb), h_dmat((size_t)(nb * nm)), h_eff((size_t)(4 * nm));

@@ -111,6 +111,7 @@ struct picca_ctx {
     double max_abs_d = 0;   // rounding of the pre-scaled coordinates of the fast loop
     bool has_log_lambda = false;
     bool has_order = false;
+    std::vector<int32_t> h_order;   // host copy of the fit orders (< 0: unprojected)
     std::vector<int64_t> h_offsets;
     std::vector<double> h_dmin, h_dmax;   // per forest: smallest / largest dist_m
     DeviceBuffer<double> z, r, d, w, dw, ll, zq;
@@ -171,8 +172,8 @@ struct picca_ctx {
     int32_t xcf_rows = 0;
     DeviceBuffer<picca::XcfPairRecord> x_rec;
     DeviceBuffer<int32_t> x_forest, x_row;
-    DeviceBuffer<int64_t> x_begin;
-    int64_t xcf_groups = 0;
+    DeviceBuffer<int64_t> x_begin, x_chunk;
+    int64_t xcf_groups = 0, xcf_chunks = 0;
     // host copies of the groups and, per pair in record order, the smallest and largest
     // r_trans any pixel of its forest can have with the quasar (for the row windows)
     std::vector<int64_t> hx_begin;
@@ -182,7 +183,8 @@ struct picca_ctx {
     struct XcfWindow {
         int32_t t0 = 0, rows = 0;
         int64_t groups = 0;
-        DeviceBuffer<int64_t> idx, begin;
+        int64_t chunks = 0;
+        DeviceBuffer<int64_t> idx, begin, chunk;
         DeviceBuffer<int32_t> forest, row;
     };
     std::vector<std::unique_ptr<XcfWindow>> x_windows;

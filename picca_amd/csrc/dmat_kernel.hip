@@ -605,6 +605,10 @@ constexpr size_t FILL_LDS_DOUBLES = (size_t)DMAT_CHUNK_RP * DMAT_CHUNK_MRP   // 
 
 }  // namespace
 
+// QUASAR: every delta2 of the batch is an unprojected quasar (order < 0), whose pairs have
+// no eta1, eta3, eta5..eta8 terms; a separate instantiation keeps the forest x forest form's
+// register budget.
+template <bool QUASAR>
 __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C, DmatForests F,
                                                                     DmatPairs P, DmatWork W,
                                                                     DmatOut O, int groups) {
@@ -668,17 +672,21 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
                 if (t < t_lo || t > t_hi || u < u_lo || u > u_hi) continue;
                 const DmatPair q = load_pair(C, F, P, s);
 
-                // rank-4 term S[., t] x eta5..8[., u] (eta already masked by U)
-                for (int k = tid; k < 4 * DMAT_CHUNK_RP; k += DMAT_FILL_BLOCK) {
-                    const int c = k / DMAT_CHUNK_RP, r = k - c * DMAT_CHUNK_RP;
-                    vS[k] = r < nrp ? W.S[(((int64_t)s * 4 + c) * C.nt + t) * C.np + rp0 + r]
+                // rank-4 term S[., t] x eta5..8[., u] (eta already masked by U); eta5..8
+                // vanish when delta2 is not projected
+                if (!QUASAR) {
+                    for (int k = tid; k < 4 * DMAT_CHUNK_RP; k += DMAT_FILL_BLOCK) {
+                        const int c = k / DMAT_CHUNK_RP, r = k - c * DMAT_CHUNK_RP;
+                        vS[k] = r < nrp
+                                    ? W.S[(((int64_t)s * 4 + c) * C.nt + t) * C.np + rp0 + r]
                                     : 0.0;
-                }
-                for (int k = tid; k < 4 * DMAT_CHUNK_MRP; k += DMAT_FILL_BLOCK) {
-                    const int c = k / DMAT_CHUNK_MRP, m = k - c * DMAT_CHUNK_MRP;
-                    vE[k] = m < nmrp
-                                ? W.eta[(((int64_t)s * 4 + c) * C.ntm + u) * C.npm + mrp0 + m]
-                                : 0.0;
+                    }
+                    for (int k = tid; k < 4 * DMAT_CHUNK_MRP; k += DMAT_FILL_BLOCK) {
+                        const int c = k / DMAT_CHUNK_MRP, m = k - c * DMAT_CHUNK_MRP;
+                        vE[k] = m < nmrp ? W.eta[(((int64_t)s * 4 + c) * C.ntm + u) * C.npm +
+                                                 mrp0 + m]
+                                         : 0.0;
+                    }
                 }
                 for (int m = tid; m < DMAT_CHUNK_MRP; m += DMAT_FILL_BLOCK) {
                     double v = 0.0;
@@ -692,6 +700,7 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
                 __syncthreads();
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
+                    if (QUASAR) break;
                     double sv[DMAT_FILL_RP];
 #pragma unroll
                     for (int a = 0; a < DMAT_FILL_RP; ++a) sv[a] = vS[c * DMAT_CHUNK_RP + ty + 16 * a];
@@ -717,7 +726,9 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
                 const double *__restrict__ Bf = F.zfb + q.ob;
 
                 // ---- pixels i of delta1: P, P' (row t) and eta1, eta3 (row u) ----------
-                {
+                // (eta1, eta3 vanish when delta2 is not projected: the QUASAR form adds the
+                // identity term with the pixels j below)
+                if (!QUASAR) {
                     const int32_t *it = W.rng_it + ((int64_t)s * C.nt + t) * 2;
                     const int32_t *iu = W.rng_iu + ((int64_t)s * C.ntm + u) * 2;
                     const int i0 = max(it[0], iu[0]);
@@ -770,8 +781,65 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
                         for (int k = lane; k < VEC_LEN; k += 64) vec[k] = 0;
                     }
                 }
+                // ---- QUASAR: delta2 is an unprojected quasar; per pixel j, the whole
+                // workgroup walks the pixels i that reach row t or row u into one shared
+                // Q, Q', eta2, eta4, and adds the identity term of each selected pair -----
+                if (QUASAR) {
+                    const int32_t *it = W.rng_it + ((int64_t)s * C.nt + t) * 2;
+                    const int32_t *iu = W.rng_iu + ((int64_t)s * C.ntm + u) * 2;
+                    const int32_t *jt = W.rng_jt + ((int64_t)s * C.nt + t) * 2;
+                    const int32_t *ju = W.rng_ju + ((int64_t)s * C.ntm + u) * 2;
+                    const int ia = min(it[0], iu[0]), ib = max(it[1], iu[1]);
+                    const int j0 = min(jt[0], ju[0]), j1 = max(jt[1], ju[1]);
+                    double *sv = vecs;   // the vectors of wave 0, shared by the workgroup
+                    for (int j = j0; ia <= ib && j <= j1; ++j) {
+                        const double w2 = Bw[j];
+                        if (w2 == 0) continue;
+                        const double r2 = Br[j], d2 = Bd[j], z2 = Bz[j];
+                        const double f2 = evol ? Bf[j] : 1.0;
+                        const bool j_zerr = q.zerr_on && zerr_cut(C, z2, q.zq1);
+                        int i_lo, i_hi;
+                        window_a(C, q, F, r2, d2, lane, i_lo, i_hi);
+                        i_lo = max(i_lo, ia);
+                        i_hi = min(i_hi, ib + 1);
+                        for (int i = i_lo + tid; i < i_hi; i += DMAT_FILL_BLOCK) {
+                            const double w1 = Aw[i];
+                            if (w1 == 0) continue;
+                            double r_par, r_trans;
+                            int A = 0, M = 0;
+                            if (!pair_bins(C, Ar[i], Ad[i], r2, d2, q.cos_half, q.sin_half,
+                                           r_par, r_trans, A, M))
+                                continue;
+                            if (M < 0 || M >= C.nm) continue;
+                            const double z1 = Az[i], l1 = Al[i];
+                            const double zfac = evol ? Af[i] * f2 : 1.0;
+                            const int mu = M % C.ntm;
+                            const int m = M / C.ntm - mrp0;
+                            const bool in_m = mu == u && m >= 0 && m < nmrp;
+                            if (in_m && mask[m] != 0) {
+                                lds_add(sv + VEC_E + m, zfac * w1 / q.sum_w1);
+                                if (q.order1) lds_add(sv + VEC_E2 + m, zfac * w1 * l1 / q.sw1);
+                            }
+                            const bool i_sel = !(q.zerr_on && zerr_cut(C, z1, q.zq2));
+                            if (!pair_selected(C, q, i_sel, j_zerr, (z1 + z2) / 2, r_par))
+                                continue;
+                            if (A < 0 || A >= C.nb || A % C.nt != t) continue;
+                            const int r = A / C.nt - rp0;
+                            if (r < 0 || r >= nrp) continue;
+                            const double w12 = w1 * w2;
+                            lds_add(sv + VEC_P + r, w12);
+                            lds_add(sv + VEC_P2 + r, w12 * l1);
+                            if (in_m) lds_add(diag + r * DMAT_CHUNK_MRP + m, w12 * zfac);
+                        }
+                        __syncthreads();
+                        outer(1);
+                        __syncthreads();
+                        for (int k = tid; k < VEC_LEN; k += DMAT_FILL_BLOCK) sv[k] = 0;
+                        __syncthreads();
+                    }
+                }
                 // ---- pixels j of delta2: Q, Q' (row t) and eta2, eta4 (row u) ----------
-                {
+                if (!QUASAR) {
                     const int32_t *jt = W.rng_jt + ((int64_t)s * C.nt + t) * 2;
                     const int32_t *ju = W.rng_ju + ((int64_t)s * C.ntm + u) * 2;
                     const int j0 = max(jt[0], ju[0]);
@@ -862,14 +930,15 @@ hipError_t launch_dmat_pass(const DmatConst &C, const DmatForests &F, const Dmat
 }
 
 hipError_t launch_dmat_fill(const DmatConst &C, const DmatForests &F, const DmatPairs &P,
-                            const DmatWork &W, const DmatOut &O, int groups,
+                            const DmatWork &W, const DmatOut &O, int groups, bool quasar,
                             hipStream_t stream) {
     if (P.count <= 0) return hipSuccess;
-    hipError_t err = hipFuncSetAttribute((const void *)dmat_fill_kernel,
+    auto kern = quasar ? dmat_fill_kernel<true> : dmat_fill_kernel<false>;
+    hipError_t err = hipFuncSetAttribute((const void *)kern,
                                          hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)dmat_fill_lds(C));
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(dmat_fill_kernel, dim3(C.nt * C.ntm, groups), dim3(DMAT_FILL_BLOCK),
+    hipLaunchKernelGGL(kern, dim3(C.nt * C.ntm, groups), dim3(DMAT_FILL_BLOCK),
                        dmat_fill_lds(C), stream, C, F, P, W, O, groups);
     return hipGetLastError();
 }

@@ -204,8 +204,9 @@ struct DmatOut {
 hipError_t launch_dmat_pass(const DmatConst &C, const DmatForests &F, const DmatPairs &P,
                             const DmatWork &W, const DmatOut &O, size_t lds,
                             hipStream_t stream);
+// quasar: every delta2 of the batch has order < 0 (the forest x quasar form)
 hipError_t launch_dmat_fill(const DmatConst &C, const DmatForests &F, const DmatPairs &P,
-                            const DmatWork &W, const DmatOut &O, int groups,
+                            const DmatWork &W, const DmatOut &O, int groups, bool quasar,
                             hipStream_t stream);
 size_t dmat_fill_lds(const DmatConst &C);
 
@@ -297,7 +298,9 @@ struct XcfView {
     const int32_t *__restrict__ forest;     // [count] forest of each group
     const int64_t *__restrict__ begin;      // [count + 1] pair range of each group
     const int32_t *__restrict__ row;        // [count] output row of each group
+    const int64_t *__restrict__ chunk;      // [nchunks + 1] first group of each claim
     int64_t count;
+    int64_t nchunks;
 };
 
 hipError_t launch_xcf(int mode, bool lds_hist, bool fast, const XiConst &C,

@@ -405,10 +405,18 @@ __global__ __launch_bounds__(XI_BLOCK) void xcf_kernel(XiConst C, ForestView F, 
             }
         }
     };
+    // thread 0's claimed run of groups [next, end): one queue ticket claims a run of groups
+    // of one row (X.chunk), so the histogram moves to HBM at most once per run
+    int64_t next = 0, end = 0;
     for (;;) {
         __syncthreads();
         if (tid == 0) {
-            s_group = (int64_t)atomicAdd(O.queue, 1ull);
+            if (next >= end) {
+                const int64_t c = (int64_t)atomicAdd(O.queue, 1ull);
+                next = c < X.nchunks ? X.chunk[c] : X.count;
+                end = c < X.nchunks ? X.chunk[c + 1] : X.count + 1;
+            }
+            s_group = next++;
             s_task = 0;
         }
         __syncthreads();
