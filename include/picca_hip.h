@@ -16,6 +16,8 @@
  *                      kernel py/picca/xcf.py:223-322
  *   picca_cont_*       compute_continuum     py/picca/delta_extraction/expected_fluxes/
  *                      utils.py:12-136 (the Migrad fit of its chi2), batched over forests
+ *   picca_metal_*      compute_metal_dmat    py/picca/cf.py:890-1232 and
+ *                      py/picca/xcf.py:677-800, one absorber pass over the kept pairs
  *   picca_neighbour_candidates  the healpy query_disc superset of fill_neighs
  *                      py/picca/cf.py:82-135 and py/picca/xcf.py:71-128
  *   picca_cf_params    the module globals    py/picca/cf.py:28-79 (None -> has_* = 0)
@@ -264,6 +266,56 @@ int picca_neighbour_candidates(int64_t n_src, const double *sx, const double *sy
                                double chord_max, double small_angle, int32_t auto_mode,
                                int64_t *counts, const int64_t *offsets, int64_t *out_target,
                                double *out_cos);
+
+/*
+ * Metal distortion matrices (compute_metal_dmat of py/picca/cf.py:890-1232 and of
+ * py/picca/xcf.py:677-800).
+ *
+ * picca_metal_set_absorber stores one absorber's view of every pixel of the forest store
+ * (store order, the store's pixel count): its redshift z = 10^log_lambda / lambda_abs - 1,
+ * r_comov(z), dist_m(z), the pixel weight with 0 where z >= the forest's z_qso (the
+ * reference drops those pixels, py/picca/cf.py:1010-1019, :1038-1047, xcf.py:722-731), and
+ * the redshift-evolution factor of the pixel: (1+z)^(alpha_abs-1) for the auto/cross forms
+ * (py/picca/cf.py:1062-1066), ((1+z)/(1+z_ref))^(alpha_abs-1) for the forest x quasar form
+ * (py/picca/xcf.py:747-749).  A quasar stored as a one-pixel forest has its own z_qso,
+ * r_comov, dist_m, weight and factor 1.  slot is 0..3.
+ */
+int picca_metal_set_absorber(picca_ctx *ctx, int32_t slot, const double *z,
+                             const double *r_comov, const double *dist_m,
+                             const double *weights, const double *zfac);
+
+#define PICCA_METAL_AUTO 0  /* |r_par|; in range when the floor / truncated bins are   */
+#define PICCA_METAL_CROSS 1 /* signed r_par, bins as PICCA_METAL_AUTO                   */
+#define PICCA_METAL_XCF 2   /* forest x quasar: r_min < r_par < r_max, r_t < r_t_max   */
+
+/* Counters of one metal pass (no counterpart in picca). */
+typedef struct picca_metal_report {
+    int64_t data_pairs;      /* pairs added to weights_dmat                            */
+    int64_t model_pairs;     /* pairs added to dmat                                    */
+    int64_t overflow_pairs;  /* PICCA_METAL_XCF bins rounded up to np (skipped; picca's
+                                bincount would fail there)                             */
+    int64_t visited_pairs;   /* pixel pairs evaluated after pruning                    */
+    double kernel_ms;        /* device time of the pass kernel (HIP events)           */
+} picca_metal_report;
+
+/*
+ * One pass of compute_metal_dmat over an explicit list of kept pairs (the rejection
+ * draws are made by the caller in reference order): delta1's pixels as the absorber of
+ * slot_a, delta2's as slot_b, data bins from the store's Lya r_comov / dist_m, model bins
+ * from the absorbers'.  The pair's evolution factor is zfac_a * zfac_b / zw_den
+ * ((1+z_ref)^(alpha_a+alpha_b-2) for the auto/cross forms, 1 for the forest x quasar
+ * form).  Uses num_bins_* / num_model_bins_*, r_par_min/max, r_trans_max and
+ * remove_same_half_plate_close_pairs of params (with PICCA_PAIR_SAME_HALF_PLATE flags).
+ * Outputs are accumulated into the caller's arrays as in picca_dmat_run (the swapped
+ * second pass of py/picca/cf.py:1092-1200 is a second call with the slots exchanged).
+ * report may be NULL.
+ */
+int picca_metal_pass(picca_ctx *ctx, const picca_cf_params *params, int32_t mode,
+                     int32_t slot_a, int32_t slot_b, double zw_den, int64_t num_pairs,
+                     const int32_t *pair_a, const int32_t *pair_b, const double *pair_ang,
+                     const uint8_t *pair_flags, double *weights_dmat, double *dmat,
+                     double *r_par_eff, double *r_trans_eff, double *z_eff,
+                     double *weight_eff, picca_metal_report *report);
 
 #ifdef __cplusplus
 }

@@ -16,6 +16,8 @@
  *   oracle_xcf_pair   <- compute_xi_forest_pairs_fast of xcf, py/picca/xcf.py:223-322
  *   oracle_xdmat_forest <- compute_dmat_forest_pairs_fast of xcf,
  *                        py/picca/xcf.py:427-658 (one delta with its kept quasars)
+ *   oracle_metal_pair <- one pass of compute_metal_dmat for one pair,
+ *                        py/picca/cf.py:890-1232 and py/picca/xcf.py:677-800
  *
  * Loop order, branch order and every floating-point operation follow the reference
  * (divide-then-multiply bin assignment, cos/sin of ang/2 from libm, sequential sums
@@ -722,4 +724,109 @@ done:
     free(all_model);
     free(uniq);
     return status;
+}
+
+
+/*
+ * One pass of the metal distortion matrix for one kept pair.
+ *
+ *   mode 0: compute_metal_dmat of the auto-correlation (py/picca/cf.py:994-1090 for the
+ *           first pass, :1092-1200 for the swapped one), |r_par|;
+ *   mode 1: the same with x_correlation (signed r_par);
+ *   mode 2: compute_metal_dmat of xcf (py/picca/xcf.py:710-790): delta x quasar.
+ *
+ * Pixel i of the first forest has its Lya coordinates (r1, d1), its weight w1 (0 where
+ * the absorber's redshift z1a is not below the quasar's: the reference drops those
+ * pixels, which changes no sum), the absorber's redshift, coordinates (z1a, r1a, d1a)
+ * and the redshift-evolution factor f1a; the same for the second forest (a quasar in
+ * mode 2: its own redshift and coordinates, f = 1).  The factor of a pair is
+ * f1a * f2a / zw_den, as the reference's
+ * (1+z1)^(a1-1) * (1+z2)^(a2-1) / (1+z_ref)^(a1+a2-2) (modes 0, 1) or
+ * ((1+z1)/(1+z_ref))^(a-1) (mode 2, with f2a = zw_den = 1).
+ *
+ * Data bins use the Lya coordinates, model bins the absorbers'.  Modes 0 and 1 keep a pair
+ * whose bins fall in range (floor of r_par, truncation of r_trans); mode 2 keeps
+ * r_min < r_par < r_max, r_trans < r_t_max for both sets of coordinates, then truncates.
+ * Same-half-plate close pairs get weight 0 (modes 0, 1).  A data bin beyond np*nt in
+ * mode 2 (rounding up at r_par_max; the reference's bincount would then fail) is skipped
+ * and counted.  stats: [0] pairs added to weights_dmat, [1] pairs added to dmat,
+ * [2] skipped overflow pairs.
+ */
+void oracle_metal_pair(const oracle_params *p, int32_t mode, double zw_den,
+                       const double *r1, const double *d1, const double *w1,
+                       const double *z1a, const double *r1a, const double *d1a,
+                       const double *f1a, int64_t n1, const double *r2, const double *d2,
+                       const double *w2, const double *z2a, const double *r2a,
+                       const double *d2a, const double *f2a, int64_t n2, double ang,
+                       int32_t same_half_plate, double *weights_dmat, double *dmat,
+                       double *r_par_eff, double *r_trans_eff, double *z_eff,
+                       double *weight_eff, int64_t *stats) {
+    const int np_ = p->num_bins_r_par, nt = p->num_bins_r_trans;
+    const int npm = p->num_model_bins_r_par, ntm = p->num_model_bins_r_trans;
+    const int64_t nb = (int64_t)np_ * nt, nm = (int64_t)npm * ntm;
+    const double span = p->r_par_max - p->r_par_min;
+    const double c = cos(ang / 2), s = sin(ang / 2);
+    const int shp = p->remove_same_half_plate_close_pairs && same_half_plate && mode != 2;
+    for (int64_t i = 0; i < n1; ++i) {
+        if (w1[i] == 0) continue;
+        for (int64_t j = 0; j < n2; ++j) {
+            if (w2[j] == 0) continue;
+            double r_par = (r1[i] - r2[j]) * c;
+            if (mode == 0) r_par = fabs(r_par);
+            const double r_trans = (d1[i] + d2[j]) * s;
+            double w12 = w1[i] * w2[j];
+            int64_t bin;
+            if (mode == 2) {
+                if (!(r_par > p->r_par_min && r_par < p->r_par_max && r_trans < p->r_trans_max))
+                    continue;
+                const int64_t bp = (int64_t)((r_par - p->r_par_min) / span * np_);
+                const int64_t bt = (int64_t)(r_trans / p->r_trans_max * nt);
+                bin = bt + (int64_t)nt * bp;
+                if (bin >= nb) {
+                    stats[2] += 1;
+                    continue;
+                }
+                if (bt < 0) continue;   /* negative r_trans: not a physical input */
+            } else {
+                const double bp = floor((r_par - p->r_par_min) / span * np_);
+                const double bt = trunc(r_trans / p->r_trans_max * nt);
+                if (!(bp < np_ && bt < nt && bp >= 0 && bt >= 0)) continue;
+                if (shp && fabs(r_par) < span / np_) w12 = 0.0;
+                bin = (int64_t)bt + (int64_t)nt * (int64_t)bp;
+            }
+            weights_dmat[bin] += w12;
+            stats[0] += 1;
+
+            double r_par_a = (r1a[i] - r2a[j]) * c;
+            if (mode == 0) r_par_a = fabs(r_par_a);
+            const double r_trans_a = (d1a[i] + d2a[j]) * s;
+            int64_t model;
+            if (mode == 2) {
+                if (!(r_par_a > p->r_par_min && r_par_a < p->r_par_max &&
+                      r_trans_a < p->r_trans_max))
+                    continue;
+                const int64_t mp = (int64_t)((r_par_a - p->r_par_min) / span * npm);
+                const int64_t mt = (int64_t)(r_trans_a / p->r_trans_max * ntm);
+                model = mt + (int64_t)ntm * mp;
+                if (model >= nm) {
+                    stats[2] += 1;
+                    continue;
+                }
+                if (mt < 0) continue;
+            } else {
+                const double mp = floor((r_par_a - p->r_par_min) / span * npm);
+                const double mt = trunc(r_trans_a / p->r_trans_max * ntm);
+                if (!(mp < npm && mt < ntm && mp >= 0 && mt >= 0)) continue;
+                model = (int64_t)mt + (int64_t)ntm * (int64_t)mp;
+            }
+            const double zw = f1a[i] * f2a[j] / zw_den;
+            const double wz = w12 * zw;
+            dmat[model + nm * bin] += wz;
+            r_par_eff[model] += r_par_a * w12 * zw;
+            r_trans_eff[model] += r_trans_a * w12 * zw;
+            z_eff[model] += (z1a[i] + z2a[j]) / 2 * w12 * zw;
+            weight_eff[model] += wz;
+            stats[1] += 1;
+        }
+    }
 }

@@ -15,6 +15,9 @@ module, and only as the checker; the engine never imports it.
 * ``compute_xi`` / ``compute_dmat`` restate the per-healpix loops of
   ``py/picca/cf.py:138-247`` and ``:390-517`` over ``Delta``-like objects, calling the C
   kernel once per pair, in the reference's order.
+* ``compute_metal_dmat`` / ``compute_metal_xdmat`` restate ``compute_metal_dmat`` of
+  ``py/picca/cf.py:890-1232`` and ``py/picca/xcf.py:677-800`` with the C pass
+  ``oracle_metal_pair``.
 """
 import ctypes
 import os
@@ -143,6 +146,11 @@ def lib():
                                              + [ctypes.c_int64] + [P] * 5 + [ctypes.c_int64]
                                              + [P] * 6 + [ctypes.c_int32, P])
         _lib.oracle_xdmat_forest.restype = ctypes.c_int
+        _lib.oracle_metal_pair.argtypes = ([ctypes.POINTER(OracleParams), ctypes.c_int32,
+                                            ctypes.c_double] + [P] * 7 + [ctypes.c_int64]
+                                           + [P] * 7 + [ctypes.c_int64, ctypes.c_double,
+                                                        ctypes.c_int32] + [P] * 7)
+        _lib.oracle_metal_pair.restype = None
     return _lib
 
 
@@ -474,3 +482,117 @@ def compute_xdmat(params, data, healpixs, neighbours, reject):
                          d1.order, stats)
     out[1] = out[1].reshape(nb, nm)
     return (out[0], out[1], out[2], out[3], out[4], out[5], num_pairs, num_pairs_used), stats
+
+
+# ---------------------------------------------------------------------------
+# metal distortion matrices (py/picca/cf.py:890-1232, py/picca/xcf.py:677-800)
+# ---------------------------------------------------------------------------
+
+def absorber_pixels(delta, lambda_abs, alpha_abs, cosmo, z_ref=None):
+    """Pixels of one delta seen as absorber `lambda_abs`: (z, r_comov, dist_m, weight with
+    the pixels whose absorber redshift is not below z_qso set to 0, evolution factor).
+    The factor is (1+z)^(alpha-1), or ((1+z)/(1+z_ref))^(alpha-1) when z_ref is given
+    (the cross-correlation's form)."""
+    z = 10**delta.log_lambda / lambda_abs - 1
+    r = cosmo.get_r_comov(z)
+    d = cosmo.get_dist_m(z)
+    w = np.where(z < delta.z_qso, delta.weights, 0.0)
+    if z_ref is None:
+        f = (1 + z) ** (alpha_abs - 1)
+    else:
+        f = ((1.0 + z) / (1.0 + z_ref)) ** (alpha_abs - 1.0)
+    return z, r, d, w, f
+
+
+def metal_pair(params, mode, zw_den, lya1, abs1, lya2, abs2, ang, same_half_plate, out,
+               stats):
+    """One pass for one pair.  lyaN = (r_comov, dist_m) of the Lya coordinates, absN =
+    absorber_pixels(...) of that forest; out = (weights_dmat, dmat (flat), r_par_eff,
+    r_trans_eff, z_eff, weight_eff)."""
+    za, ra, da, wa, fa = [_f(x) for x in abs1]
+    zb, rb, db, wb, fb = [_f(x) for x in abs2]
+    r1, d1 = _f(lya1[0]), _f(lya1[1])
+    r2, d2 = _f(lya2[0]), _f(lya2[1])
+    lib().oracle_metal_pair(ctypes.byref(params), int(mode), float(zw_den), _p(r1), _p(d1),
+                            _p(wa), _p(za), _p(ra), _p(da), _p(fa), len(r1), _p(r2), _p(d2),
+                            _p(wb), _p(zb), _p(rb), _p(db), _p(fb), len(r2), float(ang),
+                            int(same_half_plate), *[_p(x) for x in out], _p(stats))
+
+
+def compute_metal_dmat(params, data, healpixs, neighbours, reject, cosmo, abs_igm1,
+                       abs_igm2, absorbers, alpha_abs, lambda_abs, lambda_abs2=None):
+    """compute_metal_dmat of the auto/cross correlation (py/picca/cf.py:890-1232) with the
+    legacy global numpy stream.  `absorbers` maps names to rest wavelengths, `alpha_abs`
+    names to evolution exponents; params.x_correlation selects the signed form."""
+    nb = params.num_bins_r_par * params.num_bins_r_trans
+    nm = params.num_model_bins_r_par * params.num_model_bins_r_trans
+    out = [np.zeros(nb), np.zeros(nb * nm)] + [np.zeros(nm) for _ in range(4)]
+    stats = np.zeros(3, dtype=np.int64)
+    mode = 1 if params.x_correlation else 0
+    a1, a2 = alpha_abs[abs_igm1], alpha_abs[abs_igm2]
+    den = (1 + params.z_ref) ** (a1 + a2 - 2)
+    if lambda_abs2 is None:
+        lambda_abs2 = lambda_abs
+    swap = ((not params.x_correlation) and abs_igm1 != abs_igm2) or \
+        (params.x_correlation and lambda_abs == lambda_abs2)
+    num_pairs = 0
+    num_pairs_used = 0
+    for hp in healpixs:
+        for d1 in data[hp]:
+            neigh = neighbours[id(d1)]
+            w = np.random.rand(len(neigh)) > reject
+            num_pairs += len(neigh)
+            num_pairs_used += w.sum()
+            for d2, keep in zip(neigh, w):
+                if not keep:
+                    continue
+                shp = same_half_plate(d1, d2, params.remove_same_half_plate_close_pairs)
+                ang = angle(d1, d2)
+                lya1 = (d1.r_comov, d1.dist_m)
+                lya2 = (d2.r_comov, d2.dist_m)
+                metal_pair(params, mode, den, lya1,
+                           absorber_pixels(d1, absorbers[abs_igm1], a1, cosmo), lya2,
+                           absorber_pixels(d2, absorbers[abs_igm2], a2, cosmo), ang, shp,
+                           out, stats)
+                if swap:
+                    metal_pair(params, mode, den, lya1,
+                               absorber_pixels(d1, absorbers[abs_igm2], a2, cosmo), lya2,
+                               absorber_pixels(d2, absorbers[abs_igm1], a1, cosmo), ang,
+                               shp, out, stats)
+    out[1] = out[1].reshape(nb, nm)
+    return (out[0], out[1], out[2], out[3], out[4], out[5], num_pairs,
+            int(num_pairs_used)), stats
+
+
+def compute_metal_xdmat(params, data, healpixs, neighbours, reject, cosmo, abs_igm,
+                        absorbers, alpha_abs):
+    """compute_metal_dmat of the cross-correlation (py/picca/xcf.py:677-800): a delta
+    whose pixels all lie beyond its quasar's redshift as absorber `abs_igm` is skipped
+    before its draw; the quasars are one-pixel forests with evolution factor 1."""
+    nb = params.num_bins_r_par * params.num_bins_r_trans
+    nm = params.num_model_bins_r_par * params.num_model_bins_r_trans
+    out = [np.zeros(nb), np.zeros(nb * nm)] + [np.zeros(nm) for _ in range(4)]
+    stats = np.zeros(3, dtype=np.int64)
+    num_pairs = 0
+    num_pairs_used = 0
+    for hp in healpixs:
+        for d1 in data[hp]:
+            pix = absorber_pixels(d1, absorbers[abs_igm], alpha_abs[abs_igm], cosmo,
+                                  z_ref=params.z_ref)
+            if not np.any(pix[0] < d1.z_qso):
+                continue
+            objs_n, angs = neighbours[id(d1)]
+            w = np.random.rand(len(objs_n)) > reject
+            num_pairs += len(objs_n)
+            num_pairs_used += w.sum()
+            for obj, ang, keep in zip(objs_n, angs, w):
+                if not keep:
+                    continue
+                one = lambda v: np.asarray([v], dtype=np.float64)  # noqa: E731
+                qso = (one(obj.z_qso), one(obj.r_comov), one(obj.dist_m), one(obj.weights),
+                       one(1.0))
+                metal_pair(params, 2, 1.0, (d1.r_comov, d1.dist_m), pix,
+                           (qso[1], qso[2]), qso, ang, False, out, stats)
+    out[1] = out[1].reshape(nb, nm)
+    return (out[0], out[1], out[2], out[3], out[4], out[5], num_pairs,
+            int(num_pairs_used)), stats

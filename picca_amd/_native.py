@@ -20,6 +20,9 @@ PICCA_E_NOMEM = -5
 
 PICCA_PAIR_SAME_HALF_PLATE = 1
 PICCA_PAIR_XCF_COUNT = 2
+PICCA_METAL_AUTO = 0
+PICCA_METAL_CROSS = 1
+PICCA_METAL_XCF = 2
 
 PAIR_SAME_HALF_PLATE = 1
 
@@ -71,6 +74,20 @@ class XiReport(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class MetalReport(ctypes.Structure):
+    """``picca_metal_report``."""
+    _fields_ = [
+        ("data_pairs", ctypes.c_int64),
+        ("model_pairs", ctypes.c_int64),
+        ("overflow_pairs", ctypes.c_int64),
+        ("visited_pairs", ctypes.c_int64),
+        ("kernel_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
 _I64 = ctypes.c_int64
@@ -95,6 +112,9 @@ _PROTOTYPES = {
     "picca_xi_download": [_P, _P, _P, _P, _P, _P, _P],
     "picca_dmat_run": [_P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "picca_diag_copy": [_P, _I64, _P],
+    "picca_metal_set_absorber": [_P, _I32, _P, _P, _P, _P, _P],
+    "picca_metal_pass": [_P, _P, _I32, _I32, _I32, _F64, _I64, _P, _P, _P, _P, _P, _P, _P, _P,
+                         _P, _P, _P],
     "picca_xcf_set_pairs": [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _I32, _I32],
     "picca_xcf_execute": [_P, _P, _P],
     "picca_cont_set_forests": [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P],

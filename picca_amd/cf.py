@@ -20,7 +20,9 @@ There is no CPU path: without the HIP library every call raises ``RuntimeError``
 """
 import numpy as np
 
+from . import _native
 from . import engine as _engine
+from . import metal as _metal
 from .data import angle_between
 from .neighbours import find_neighbours
 from .store import ForestStore
@@ -389,6 +391,88 @@ def compute_dmat(healpixs):
     last_report = {"in_range_pairs": int(stats[0]), "selected_pairs": int(stats[1]),
                    "beyond_count_pass": int(stats[2]), "overflow_pairs": int(stats[3])}
     return (weights_dmat, dmat, r_par_eff, r_trans_eff, z_eff, weight_eff, num_pairs,
+            num_pairs_used)
+
+
+# ---------------------------------------------------------------------------
+# metal distortion matrix
+# ---------------------------------------------------------------------------
+
+def build_metal_pairs(healpixs):
+    """Kept pairs of compute_metal_dmat (``py/picca/cf.py:932-952``): one draw of numpy's
+    global legacy stream per neighbour of each delta1, in healpix then list order; the
+    neighbour lists are consumed.  Returns (catalogue, pair_a, pair_b, pair_ang,
+    pair_flags, num_pairs, num_pairs_used)."""
+    cat = _catalogue()
+    pair_a, pair_b, pair_ang, pair_flags = [], [], [], []
+    num_pairs = 0
+    num_pairs_used = 0
+    for healpix in healpixs:
+        for delta1 in data[healpix]:
+            _tick()
+            neighbours = delta1.neighbours
+            if neighbours is None:
+                neighbours = []
+            keep = np.random.rand(len(neighbours)) > reject
+            num_pairs += len(neighbours)
+            num_pairs_used += keep.sum()
+            kept = [d for d, k in zip(neighbours, keep) if k]
+            if kept:
+                flags = np.array([_same_half_plate(delta1, d) for d in kept], dtype=np.uint8)
+                ang = angle_between(delta1.x_cart, delta1.y_cart, delta1.z_cart, delta1.ra,
+                                    delta1.dec, delta1.cos_dec,
+                                    np.array([d.x_cart for d in kept]),
+                                    np.array([d.y_cart for d in kept]),
+                                    np.array([d.z_cart for d in kept]),
+                                    np.array([d.ra for d in kept]),
+                                    np.array([d.dec for d in kept]))
+                pair_a.append(np.full(len(kept), cat.index_of(delta1), dtype=np.int64))
+                pair_b.append(np.array([cat.index_of(d) for d in kept], dtype=np.int64))
+                pair_ang.append(np.asarray(ang, dtype=np.float64))
+                pair_flags.append(flags)
+            delta1.neighbours = None
+    arrays = [np.concatenate(x) if x else np.zeros(0) for x in
+              (pair_a, pair_b, pair_ang, pair_flags)]
+    return (cat, arrays[0].astype(np.int64), arrays[1].astype(np.int64),
+            arrays[2].astype(np.float64), arrays[3].astype(np.uint8), num_pairs,
+            int(num_pairs_used))
+
+
+def compute_metal_dmat(healpixs, abs_igm1="LYA", abs_igm2="SiIII(1207)"):
+    """Metal distortion matrix of the pairs of ``healpixs`` (``py/picca/cf.py:890-1232``):
+    data bins from the Lya coordinates, model bins from delta1's pixels as ``abs_igm1``
+    and delta2's as ``abs_igm2``, plus the swapped pass when the reference adds it
+    (``:1092-1094``).  Uses the globals alpha_abs, cosmo, z_ref, lambda_abs(2).
+    Returns (weights_dmat, dmat [nb, nm], r_par_eff, r_trans_eff, z_eff, weight_eff,
+    num_pairs, num_pairs_used)."""
+    global last_report
+    from .constants import ABSORBER_IGM
+    cat, pair_a, pair_b, pair_ang, flags, num_pairs, num_pairs_used = build_metal_pairs(
+        healpixs)
+    params = current_params()
+    nb = num_bins_r_par * num_bins_r_trans
+    nm = num_model_bins_r_par * num_model_bins_r_trans
+    out = _metal.outputs(nb, nm)
+    eng = _engine.get_engine(device)
+    eng.set_forests(cat.store, key=cat.key)
+    alpha1, alpha2_ = alpha_abs[abs_igm1], alpha_abs[abs_igm2]
+    eng.metal_set_absorber(0, *_metal.absorber_table(cat.store, ABSORBER_IGM[abs_igm1],
+                                                     alpha1, cosmo))
+    slot2 = 0
+    if abs_igm2 != abs_igm1:
+        slot2 = 1
+        eng.metal_set_absorber(1, *_metal.absorber_table(cat.store, ABSORBER_IGM[abs_igm2],
+                                                         alpha2_, cosmo))
+    den = (1 + z_ref) ** (alpha1 + alpha2_ - 2)
+    mode = _native.PICCA_METAL_CROSS if x_correlation else _native.PICCA_METAL_AUTO
+    reports = [eng.metal_pass(params, mode, 0, slot2, den, pair_a, pair_b, pair_ang, flags,
+                              out)]
+    if ((not x_correlation) and (abs_igm1 != abs_igm2)) or (
+            x_correlation and (lambda_abs == lambda_abs2)):
+        reports.append(eng.metal_pass(params, mode, slot2, 0, den, pair_a, pair_b, pair_ang,
+                                      flags, out))
+    last_report = {"passes": reports}
+    return (out[0], out[1].reshape(nb, nm), out[2], out[3], out[4], out[5], num_pairs,
             num_pairs_used)
 
 

@@ -268,6 +268,7 @@ int picca_set_forests(picca_ctx *ctx, int64_t num_forests, const int64_t *offset
     PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     ctx->num_forests = num_forests;
     ctx->num_pixels = npix;
+    for (picca_ctx::MetalSlot &m : ctx->metal) m.set = false;   // tables of the old store
     return PICCA_OK;
 }
 

@@ -147,6 +147,17 @@ struct picca_ctx {
     DeviceBuffer<unsigned long long> ws_touched, dstats;
     DeviceBuffer<double> o_wdm, o_dmat, o_eff;
 
+    // metal distortion matrix: absorber tables per slot (store order) and per-pass tasks
+    struct MetalSlot {
+        bool set = false;
+        int64_t pixels = 0;
+        DeviceBuffer<double> z, r, d, w, f;
+    };
+    MetalSlot metal[4];
+    DeviceBuffer<int64_t> m_task_begin;
+    DeviceBuffer<int32_t> m_task_pair;
+    DeviceBuffer<unsigned long long> m_stats;
+
     // continuum fit: resident spectra (forest pixels) and per-forest results
     int64_t cont_forests = 0;
     int64_t cont_pixels = 0;

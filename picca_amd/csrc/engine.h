@@ -211,6 +211,71 @@ hipError_t launch_dmat_fill(const DmatConst &C, const DmatForests &F, const Dmat
 size_t dmat_fill_lds(const DmatConst &C);
 
 // ---------------------------------------------------------------------------------------
+// metal distortion matrix
+// ---------------------------------------------------------------------------------------
+
+constexpr int METAL_BLOCK = 1024;   // threads per workgroup (one per CU, persistent)
+
+enum : int {
+    METAL_AUTO = 0,    // |r_par|, bins by floor / truncation, in-range by bin index
+    METAL_CROSS = 1,   // signed r_par, otherwise as METAL_AUTO
+    METAL_XCF = 2,     // forest x quasar: r_min < r_par < r_max, r_trans < r_t_max
+};
+
+enum : int {
+    METAL_STAT_DATA = 0,      // pairs added to weights_dmat
+    METAL_STAT_MODEL = 1,     // pairs added to dmat
+    METAL_STAT_OVERFLOW = 2,  // METAL_XCF pairs whose bin rounds to np (skipped)
+    METAL_STAT_VISITED = 3,   // pixel pairs evaluated after pruning
+    METAL_NSTATS = 4,
+};
+
+// One absorber's view of every pixel of the store: its redshift, comoving distance,
+// transverse distance, the pixel weight (0 where the absorber's redshift is not below the
+// quasar's) and the redshift-evolution factor.
+struct MetalTable {
+    const double *__restrict__ z;
+    const double *__restrict__ r;
+    const double *__restrict__ d;
+    const double *__restrict__ w;
+    const double *__restrict__ f;
+};
+
+struct MetalConst {
+    int32_t np, nt, nb;
+    int32_t npm, ntm, nm;
+    int32_t mode;       // METAL_*
+    int32_t shp;        // remove same-half-plate close pairs
+    double rpmin, rpmax, rtmax, span, width_p;
+    double inv_den;     // 1 / the evolution factor's denominator
+};
+
+struct MetalPairs {
+    const int32_t *__restrict__ a;
+    const int32_t *__restrict__ b;
+    const double *__restrict__ cs;          // cos(ang/2), sin(ang/2)
+    const uint8_t *__restrict__ flags;      // PAIR_SAME_HALF_PLATE
+    const int64_t *__restrict__ task_begin; // [count] first 64-pixel task of each pair
+    int64_t count;
+    int64_t tasks;                          // task_pair[t]: the pair of task t
+};
+
+struct MetalOut {
+    double *weights_dmat;   // [nb]
+    double *dmat;           // [nb][nm]
+    double *r_par_eff, *r_trans_eff, *z_eff, *weight_eff;  // [nm]
+    unsigned long long *queue;
+    unsigned long long *stats;   // [METAL_NSTATS]
+};
+
+// lds_hist: weights_dmat and the four effective-coordinate sums are privatised in LDS
+hipError_t launch_metal(const MetalConst &C, const int64_t *off, const uint8_t *mono,
+                        const double *r, const double *d, const MetalTable &A,
+                        const MetalTable &B, const MetalPairs &P, const int32_t *task_pair,
+                        const MetalOut &O, bool lds_hist, int grid, hipStream_t stream);
+size_t metal_lds_bytes(const MetalConst &C);
+
+// ---------------------------------------------------------------------------------------
 // continuum fit
 // ---------------------------------------------------------------------------------------
 

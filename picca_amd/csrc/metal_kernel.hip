@@ -1,0 +1,268 @@
+// Metal distortion matrices on gfx950 (MI355X): compute_metal_dmat of picca's auto/cross
+// correlation (py/picca/cf.py:890-1232) and of the forest x quasar correlation
+// (py/picca/xcf.py:677-800), one pass (one absorber on delta1, one on delta2) over a list of
+// kept pairs.
+//
+// For every pixel pair the reference takes the Lya coordinates for the data bin A and the
+// absorbers' coordinates for the model bin M, and adds
+//     weights_dmat[A] += w1 w2,
+//     dmat[A, M] += w1 w2 f,  r_par_eff[M] += r_par' w1 w2 f,  r_trans_eff[M] += ...,
+//     z_eff[M] += (z1' + z2') / 2 w1 w2 f,  weight_eff[M] += w1 w2 f,
+// where f is the pair's redshift-evolution factor, A must be in range and M in range for
+// the second group.  It is a histogram with no continuum-projection terms, so the kernel is
+// a pair loop like the correlation's: one wave per (pair, 64 pixels of delta1), lanes over
+// the pixels i, a scalar walk over the pixels j of delta2 inside the window that can give an
+// in-range data bin.  Consecutive pairs of a lane with the same (A, M) merge into one run;
+// a run adds to weights_dmat and the four effective sums in LDS (privatised per workgroup
+// when they fit) and to dmat in HBM.  All arithmetic is FP64 with the reference's
+// divide-then-multiply bins; pixel weights are 0 where the absorber's redshift is not below
+// the quasar's (the reference drops those pixels).
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "device_util.h"
+#include "engine.h"
+
+namespace picca {
+namespace {
+
+struct MetalRun {
+    int a, m;   // data bin, model bin (-1: out of range) of the run; a < 0: no run
+    double w, wz, rp, rt, z;
+};
+
+__device__ __forceinline__ void lds_add(double *p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <bool LDS_HIST>
+__device__ __forceinline__ void metal_flush(const MetalRun &run, const MetalConst &C,
+                                            double *h_wdm, double *h_eff, const MetalOut &O) {
+    if (run.a < 0) return;
+    if (run.w != 0) {
+        if (LDS_HIST)
+            lds_add(h_wdm + run.a, run.w);
+        else
+            unsafeAtomicAdd(O.weights_dmat + run.a, run.w);
+    }
+    if (run.m < 0 || run.wz == 0) return;
+    unsafeAtomicAdd(O.dmat + (int64_t)run.a * C.nm + run.m, run.wz);
+    if (LDS_HIST) {
+        lds_add(h_eff + run.m, run.rp);
+        lds_add(h_eff + C.nm + run.m, run.rt);
+        lds_add(h_eff + 2 * C.nm + run.m, run.z);
+        lds_add(h_eff + 3 * C.nm + run.m, run.wz);
+    } else {
+        unsafeAtomicAdd(O.r_par_eff + run.m, run.rp);
+        unsafeAtomicAdd(O.r_trans_eff + run.m, run.rt);
+        unsafeAtomicAdd(O.z_eff + run.m, run.z);
+        unsafeAtomicAdd(O.weight_eff + run.m, run.wz);
+    }
+}
+
+// Flat bin of (r_par, r_trans) with n_par x n_trans bins, or -1 when out of range
+// (py/picca/cf.py:1036-1052 / py/picca/xcf.py:733-742).  METAL_XCF counts a bin that
+// rounds up to n_par (the reference's bincount would fail there) in `over`.
+template <int MODE>
+__device__ __forceinline__ int metal_bin(double r_par, double r_trans, const MetalConst &C,
+                                         int n_par, int n_trans, uint64_t &over) {
+    if (MODE == METAL_XCF) {
+        if (!(r_par > C.rpmin && r_par < C.rpmax && r_trans < C.rtmax)) return -1;
+        const int bp = (int)((r_par - C.rpmin) / C.span * n_par);
+        const int bt = (int)(r_trans / C.rtmax * n_trans);
+        const int bin = bt + n_trans * bp;
+        if (bin >= n_par * n_trans) {
+            ++over;
+            return -1;
+        }
+        return bt >= 0 ? bin : -1;   // a negative r_trans (never for physical inputs)
+    } else {
+        const double bp = floor((r_par - C.rpmin) / C.span * n_par);
+        const double bt = trunc(r_trans / C.rtmax * n_trans);
+        if (!(bp < n_par && bt < n_trans && bp >= 0 && bt >= 0)) return -1;
+        return (int)bt + n_trans * (int)bp;
+    }
+}
+
+}  // namespace
+
+template <int MODE, bool LDS_HIST>
+__global__ __launch_bounds__(METAL_BLOCK) void metal_kernel(
+    MetalConst C, const int64_t *__restrict__ off, const uint8_t *__restrict__ mono,
+    const double *__restrict__ Fr, const double *__restrict__ Fd, MetalTable TA,
+    MetalTable TB, MetalPairs P, const int32_t *__restrict__ task_pair, MetalOut O) {
+    extern __shared__ __align__(16) double hist[];   // [nb] weights_dmat, [4][nm] eff
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    double *h_wdm = hist;
+    double *h_eff = hist + C.nb;
+    if (LDS_HIST) {
+        for (int k = tid; k < C.nb + 4 * C.nm; k += METAL_BLOCK) hist[k] = 0.0;
+        __syncthreads();
+    }
+    uint64_t n_data = 0, n_model = 0, n_over = 0, n_visit = 0;
+    for (;;) {
+        int64_t t = 0;
+        if (lane == 0) t = (int64_t)atomicAdd(O.queue, 1ull);
+        t = uniform_i64(t);
+        if (t >= P.tasks) break;
+        const int s = uniform_int(task_pair[t]);
+        const int chunk = (int)(t - uniform_i64(P.task_begin[s]));
+        const int a = uniform_int(P.a[s]);
+        const int b = uniform_int(P.b[s]);
+        const int64_t oa = uniform_i64(off[a]);
+        const int la = (int)(uniform_i64(off[a + 1]) - oa);
+        const int64_t ob = uniform_i64(off[b]);
+        const int lb = (int)(uniform_i64(off[b + 1]) - ob);
+        const double cth = uniform_f64(P.cs[2 * s]);
+        const double sth = uniform_f64(P.cs[2 * s + 1]);
+        const bool shp = MODE != METAL_XCF && C.shp && (P.flags[s] & PAIR_SAME_HALF_PLATE);
+
+        const int i = chunk * 64 + lane;
+        double r1 = 0, d1 = 0, w1 = 0, z1a = 0, r1a = 0, d1a = 0, f1a = 0;
+        if (i < la) {
+            const int64_t p = oa + i;
+            r1 = Fr[p];
+            d1 = Fd[p];
+            w1 = TA.w[p];
+            z1a = TA.z[p];
+            r1a = TA.r[p];
+            d1a = TA.d[p];
+            f1a = TA.f[p];
+        }
+        const bool live = i < la && w1 != 0;
+        if (__ballot(live) == 0) continue;
+
+        // pixels j of delta2 that can give an in-range data bin with a live lane
+        int j_lo = 0, j_hi = lb;
+        const int mb = uniform_int((int)mono[b]);
+        if (lb > 1 && (mb & 1)) {
+            const double rmin = wave_min(live ? r1 : INFINITY);
+            const double rmax = wave_max(live ? r1 : -INFINITY);
+            double lo_r = rmin - C.rpmax / cth;
+            double hi_r = MODE == METAL_AUTO ? rmax + C.rpmax / cth : rmax - C.rpmin / cth;
+            lo_r -= 1e-6 + 1e-12 * fabs(lo_r);
+            hi_r += 1e-6 + 1e-12 * fabs(hi_r);
+            j_lo = uniform_int(wave_search<true>(Fr + ob, lb, lo_r, lane));
+            j_hi = uniform_int(wave_search<false>(Fr + ob, lb, hi_r, lane));
+            if ((mb & 2) && sth > 0) {
+                const double dmin = wave_min(live ? d1 : INFINITY);
+                double d_lim = C.rtmax / sth - dmin;
+                d_lim += 1e-6 + 1e-12 * fabs(d_lim);
+                j_hi = min(j_hi, uniform_int(wave_search<false>(Fd + ob, lb, d_lim, lane)));
+            }
+        }
+
+        MetalRun run{-1, -1, 0, 0, 0, 0, 0};
+        for (int j = j_lo; j < j_hi; ++j) {
+            const int64_t q = ob + j;
+            const double w2 = TB.w[q];
+            if (w2 == 0) continue;
+            const double r2 = Fr[q], d2 = Fd[q];
+            const double z2a = TB.z[q], r2a = TB.r[q], d2a = TB.d[q];
+            const double g2 = TB.f[q] * C.inv_den;
+            if (!live) continue;
+            ++n_visit;
+            double r_par = (r1 - r2) * cth;
+            if (MODE == METAL_AUTO) r_par = fabs(r_par);
+            const double r_trans = (d1 + d2) * sth;
+            const int A = metal_bin<MODE>(r_par, r_trans, C, C.np, C.nt, n_over);
+            if (A < 0) continue;
+            ++n_data;
+            double w12 = w1 * w2;
+            if (shp && fabs(r_par) < C.width_p) w12 = 0.0;
+            double r_par_a = (r1a - r2a) * cth;
+            if (MODE == METAL_AUTO) r_par_a = fabs(r_par_a);
+            const double r_trans_a = (d1a + d2a) * sth;
+            const int M = metal_bin<MODE>(r_par_a, r_trans_a, C, C.npm, C.ntm, n_over);
+            if (A != run.a || M != run.m) {
+                metal_flush<LDS_HIST>(run, C, h_wdm, h_eff, O);
+                run = MetalRun{A, M, 0, 0, 0, 0, 0};
+            }
+            run.w += w12;
+            if (M < 0) continue;
+            ++n_model;
+            const double zw = f1a * g2;
+            run.wz += w12 * zw;
+            run.rp += r_par_a * w12 * zw;
+            run.rt += r_trans_a * w12 * zw;
+            run.z += (z1a + z2a) / 2 * w12 * zw;
+        }
+        metal_flush<LDS_HIST>(run, C, h_wdm, h_eff, O);
+    }
+
+    const uint64_t v_data = wave_sum_u64(n_data);
+    const uint64_t v_model = wave_sum_u64(n_model);
+    const uint64_t v_over = wave_sum_u64(n_over);
+    const uint64_t v_visit = wave_sum_u64(n_visit);
+    if (lane == 0) {
+        atomicAdd(O.stats + METAL_STAT_DATA, (unsigned long long)v_data);
+        atomicAdd(O.stats + METAL_STAT_MODEL, (unsigned long long)v_model);
+        atomicAdd(O.stats + METAL_STAT_OVERFLOW, (unsigned long long)v_over);
+        atomicAdd(O.stats + METAL_STAT_VISITED, (unsigned long long)v_visit);
+    }
+    if (LDS_HIST) {
+        __syncthreads();
+        for (int k = tid; k < C.nb; k += METAL_BLOCK)
+            if (h_wdm[k] != 0) unsafeAtomicAdd(O.weights_dmat + k, h_wdm[k]);
+        double *eff[4] = {O.r_par_eff, O.r_trans_eff, O.z_eff, O.weight_eff};
+        for (int k = tid; k < 4 * C.nm; k += METAL_BLOCK) {
+            const double v = h_eff[k];
+            if (v != 0) unsafeAtomicAdd(eff[k / C.nm] + (k % C.nm), v);
+        }
+    }
+}
+
+size_t metal_lds_bytes(const MetalConst &C) {
+    return sizeof(double) * ((size_t)C.nb + 4 * (size_t)C.nm);
+}
+
+template <int MODE, bool LDS_HIST>
+static hipError_t launch_metal_t(const MetalConst &C, const int64_t *off, const uint8_t *mono,
+                                 const double *r, const double *d, const MetalTable &A,
+                                 const MetalTable &B, const MetalPairs &P,
+                                 const int32_t *task_pair, const MetalOut &O, int grid,
+                                 hipStream_t stream) {
+    auto kern = metal_kernel<MODE, LDS_HIST>;
+    const size_t lds = LDS_HIST ? metal_lds_bytes(C) : 0;
+    if (LDS_HIST) {
+        hipError_t err = hipFuncSetAttribute((const void *)kern,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)lds);
+        if (err != hipSuccess) return err;
+    }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(METAL_BLOCK), lds, stream, C, off, mono, r, d, A,
+                       B, P, task_pair, O);
+    return hipGetLastError();
+}
+
+hipError_t launch_metal(const MetalConst &C, const int64_t *off, const uint8_t *mono,
+                        const double *r, const double *d, const MetalTable &A,
+                        const MetalTable &B, const MetalPairs &P, const int32_t *task_pair,
+                        const MetalOut &O, bool lds_hist, int grid, hipStream_t stream) {
+    if (P.tasks <= 0) return hipSuccess;
+    switch (C.mode * 2 + (lds_hist ? 1 : 0)) {
+        case METAL_AUTO * 2:
+            return launch_metal_t<METAL_AUTO, false>(C, off, mono, r, d, A, B, P, task_pair, O,
+                                                     grid, stream);
+        case METAL_AUTO * 2 + 1:
+            return launch_metal_t<METAL_AUTO, true>(C, off, mono, r, d, A, B, P, task_pair, O,
+                                                    grid, stream);
+        case METAL_CROSS * 2:
+            return launch_metal_t<METAL_CROSS, false>(C, off, mono, r, d, A, B, P, task_pair,
+                                                      O, grid, stream);
+        case METAL_CROSS * 2 + 1:
+            return launch_metal_t<METAL_CROSS, true>(C, off, mono, r, d, A, B, P, task_pair,
+                                                     O, grid, stream);
+        case METAL_XCF * 2:
+            return launch_metal_t<METAL_XCF, false>(C, off, mono, r, d, A, B, P, task_pair, O,
+                                                    grid, stream);
+        case METAL_XCF * 2 + 1:
+            return launch_metal_t<METAL_XCF, true>(C, off, mono, r, d, A, B, P, task_pair, O,
+                                                   grid, stream);
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace picca

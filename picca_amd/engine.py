@@ -238,6 +238,35 @@ class Engine:
                 stats)
 
 
+    # -- metal distortion matrix -------------------------------------------------
+    def metal_set_absorber(self, slot, z, r_comov, dist_m, weights, zfac):
+        """One absorber's view of every pixel of the resident store (store order)."""
+        arrays = [as_f64(a) for a in (z, r_comov, dist_m, weights, zfac)]
+        check(self._lib.picca_metal_set_absorber(self._ctx, int(slot),
+                                                 *(ptr(a) for a in arrays)),
+              "picca_metal_set_absorber")
+
+    def metal_pass(self, params, mode, slot_a, slot_b, zw_den, pair_a, pair_b, pair_ang,
+                   pair_flags, out):
+        """One absorber pass over the kept pairs, accumulated into
+        out = (weights_dmat, dmat (flat), r_par_eff, r_trans_eff, z_eff, weight_eff).
+        Returns the pass report as a dict."""
+        a = as_i32(pair_a)
+        b = as_i32(pair_b)
+        ang = as_f64(pair_ang)
+        flags = as_u8(pair_flags)
+        for arr in out:
+            if arr.dtype != np.float64 or not arr.flags["C_CONTIGUOUS"]:
+                raise ValueError("metal outputs must be contiguous float64 arrays")
+        report = _native.MetalReport()
+        check(self._lib.picca_metal_pass(self._ctx, ctypes.byref(params), int(mode),
+                                         int(slot_a), int(slot_b), float(zw_den), len(a),
+                                         ptr(a), ptr(b), ptr(ang), ptr(flags),
+                                         *(ptr(x) for x in out), ctypes.byref(report)),
+              "picca_metal_pass")
+        return report.as_dict()
+
+
 _ENGINES = {}
 
 

@@ -15,7 +15,9 @@ Module globals with picca's names (py/picca/xcf.py:20-67), and:
   of the given healpix pixels in one batched GPU computation, and its per-forest kernel
   ``compute_dmat_forest_pairs_fast`` (:427-658).  Each quasar enters the engine's forest
   store as a one-pixel forest with order -1 (not projected), so the distortion-matrix
-  kernels of the auto-correlation keep only the terms of the delta's continuum fit.
+  kernels of the auto-correlation keep only the terms of the delta's continuum fit;
+* ``compute_metal_dmat`` (:677-800): the metal distortion matrix of one absorber, through
+  the engine's metal pass over the same store.
 
 The neighbour candidates come from the engine's grid search; the angles are
 ``QSO.get_angle_between`` evaluated with numpy exactly as the reference does.
@@ -26,6 +28,7 @@ import numpy as np
 
 from . import engine as _engine
 from . import _native
+from . import metal as _metal
 from .constants import SPEED_LIGHT
 from .neighbours import find_neighbours
 from .store import ForestStore
@@ -52,6 +55,7 @@ alpha = None
 alpha_obj = None
 lambda_abs = None
 alpha_abs = None
+cosmo = None
 data = None
 objs = None
 reject = None
@@ -379,3 +383,59 @@ def compute_dmat_forest_pairs_fast(log_lambda1, r_comov1, dist_m1, z1, weights1,
     r_trans_eff += out[3]
     z_eff += out[4]
     weight_eff += out[5]
+
+
+def compute_metal_dmat(healpixs, abs_igm="SiII(1526)"):
+    """Metal distortion matrix of the deltas of ``healpixs`` against their quasar
+    neighbours (``py/picca/xcf.py:677-800``) for absorber ``abs_igm`` on the forest side:
+    a delta none of whose pixels lies below its quasar's redshift as ``abs_igm`` is skipped
+    before its draw; then one draw of numpy's global legacy stream per quasar neighbour.
+    Uses the globals alpha_abs, cosmo, z_ref.  Returns (weights_dmat, dmat [nb, nm],
+    r_par_eff, r_trans_eff, z_eff, weight_eff, num_pairs, num_pairs_used)."""
+    global last_report
+    from .constants import ABSORBER_IGM
+    cat = _catalogue()
+    store, nf = _dmat_store(cat)
+    lam = ABSORBER_IGM[abs_igm]
+    forest_tab = _metal.absorber_table(cat.store, lam, alpha_abs[abs_igm], cosmo,
+                                       z_ref=z_ref)
+    obj = cat.objects
+    qso_tab = _metal.quasar_table(obj.z_qso, obj.r_comov, obj.dist_m, obj.weights)
+    table = [np.concatenate([x, y]) for x, y in zip(forest_tab, qso_tab)]
+    # pixels of each delta whose absorber redshift lies below its quasar's
+    offsets = cat.store.offsets
+    mask = forest_tab[0] < np.repeat(cat.store.z_qso, np.diff(offsets))
+    cum = np.concatenate([[0], np.cumsum(mask, dtype=np.int64)])
+    below = cum[offsets[1:]] - cum[offsets[:-1]]
+    fa, qb, angs = [], [], []
+    num_pairs = 0
+    num_pairs_used = 0
+    for healpix in healpixs:
+        for delta1 in data[healpix]:
+            k = cat.index_of(delta1)
+            if below[k] == 0:
+                continue
+            idx = getattr(delta1, "neighbour_index", None)
+            n = 0 if idx is None or delta1.neighbours is None else len(idx)
+            keep = np.random.rand(n) > reject
+            num_pairs += n
+            num_pairs_used += int(keep.sum())
+            if keep.any():
+                fa.append(np.full(int(keep.sum()), k, dtype=np.int64))
+                qb.append(np.asarray(idx, dtype=np.int64)[keep])
+                angs.append(np.asarray(delta1.neighbour_angles, dtype=np.float64)[keep])
+            delta1.neighbours = None
+    fa, qb, ang = [np.concatenate(x) if x else np.zeros(0) for x in (fa, qb, angs)]
+    nb = num_bins_r_par * num_bins_r_trans
+    nm = num_model_bins_r_par * num_model_bins_r_trans
+    out = _metal.outputs(nb, nm)
+    eng = _engine.get_engine(device)
+    eng.set_forests(store, key=("xdmat",) + cat.key)
+    eng.metal_set_absorber(0, *table)
+    params = dmat_params()
+    report = eng.metal_pass(params, _native.PICCA_METAL_XCF, 0, 0, 1.0,
+                            fa.astype(np.int64), (nf + qb).astype(np.int64),
+                            ang.astype(np.float64), np.zeros(len(fa), dtype=np.uint8), out)
+    last_report = {"passes": [report]}
+    return (out[0], out[1].reshape(nb, nm), out[2], out[3], out[4], out[5], num_pairs,
+            num_pairs_used)
