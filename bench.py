@@ -134,11 +134,13 @@ def main():
                         help="HBM bytes per launch of the pair kernel (default: the "
                              "counter-derived figure in profiles/ for this workload)")
     parser.add_argument("--mode", default="xi",
-                        choices=("xi", "dmat", "continuum", "precision", "xcf", "xdmat"),
+                        choices=("xi", "dmat", "continuum", "precision", "xcf", "xdmat",
+                                 "metal"),
                         help="xi: the pair kernel (default); dmat: the distortion matrix; "
                              "continuum: plain vs learned-seed continuum fits; precision: "
                              "xi error and edge pairs on the heavy-tailed survey; xcf / xdmat: "
-                             "forest x quasar correlation / distortion matrix")
+                             "forest x quasar correlation / distortion matrix; metal: the "
+                             "LYA x SiIII(1207) metal distortion matrix")
     parser.add_argument("--reject", type=float, default=0.99,
                         help="dmat: fraction of neighbour pairs rejected")
     parser.add_argument("--spectra", type=int, default=200000,
@@ -157,6 +159,8 @@ def main():
         return bench_precision(args)
     if args.mode in ("xcf", "xdmat"):
         return bench_xcf(args)
+    if args.mode == "metal":
+        return bench_metal(args)
     return bench_xi(args)
 
 
@@ -315,6 +319,11 @@ class DmatGlobals(Globals):
     num_model_bins_r_trans = 50
 
 
+class MetalGlobals(Globals):
+    num_model_bins_r_par = 50
+    num_model_bins_r_trans = 50
+
+
 def bench_dmat(args):
     """Distortion matrix of the 100k-forest survey: neighbour pairs kept with probability
     1 - reject (one seeded legacy-numpy draw per pair, in pair-list order), 50x50 data bins,
@@ -378,6 +387,132 @@ def bench_dmat(args):
                       % (len(sel), int(ref_stats[0])), "seconds": cpu_s}
         result["parity_sample"] = {"dmat_max_abs_err_over_max": err,
                                    "counts_equal": bool(list(mine[6]) == list(ref_stats))}
+    print(json.dumps(result))
+
+
+# FP64 operations of one pass of compute_metal_dmat per pixel pair
+# (py/picca/cf.py:1026-1090): Lya separations (sub, mul, add, mul) and data bins (sub, div,
+# mul, div, mul) for every evaluated pair; absorber separations, model bins and w1 w2 for a
+# pair with an in-range data bin; the evolution factor, the five weighted terms and their
+# five sums for a pair with an in-range model bin.
+METAL_FLOPS_VISITED = 9
+METAL_FLOPS_DATA = 10
+METAL_FLOPS_MODEL = 15
+
+
+def bench_metal(args):
+    """Metal distortion matrix LYA x SiIII(1207) of the 100k-forest survey: neighbour pairs
+    kept with probability 1 - reject (one seeded legacy-numpy draw per pair), 50x50 data
+    and model bins, both absorber passes of the auto-correlation."""
+    from picca_amd import batch, metal, synth
+    from picca_amd import _native
+    from picca_amd.constants import ABSORBER_IGM, Cosmo
+    from picca_amd.engine import device_info, get_engine, params_from_globals
+    from picca_amd.utils import compute_ang_max
+
+    survey = synth.make_survey("survey_100k", seed=args.seed, scale=args.scale)
+    store = survey.store
+    cosmo = Cosmo(0.315, verbose=False)
+    ang_max = compute_ang_max(cosmo, Globals.r_trans_max, survey.z_min)
+    pairs = batch.xi_pairs(store, survey.healpix, ang_max)
+    keep = np.random.RandomState(args.seed).rand(pairs.num_pairs) > args.reject
+    a, b = pairs.pair_a[keep], pairs.pair_b[keep]
+    ang, flags = pairs.pair_ang[keep], pairs.pair_flags[keep]
+    log("metal: %d of %d neighbour pairs kept" % (len(a), pairs.num_pairs))
+    t = time.time()
+    lya = metal.absorber_table(store, ABSORBER_IGM["LYA"], 2.9, cosmo)
+    si = metal.absorber_table(store, ABSORBER_IGM["SiIII(1207)"], 1.0, cosmo)
+    den = (1 + MetalGlobals.z_ref) ** (2.9 + 1.0 - 2)
+    log("metal: absorber tables (%.1f s)" % (time.time() - t))
+    params = params_from_globals(MetalGlobals)
+    mode = _native.PICCA_METAL_AUTO
+    eng = get_engine(0)
+    info = device_info(0)
+    eng.set_forests(store, key="bench_metal")
+    eng.metal_set_absorber(0, *lya)
+    eng.metal_set_absorber(1, *si)
+    nb = MetalGlobals.num_bins_r_par * MetalGlobals.num_bins_r_trans
+    nm = MetalGlobals.num_model_bins_r_par * MetalGlobals.num_model_bins_r_trans
+
+    def step():
+        out = metal.outputs(nb, nm)
+        reps = [eng.metal_pass(params, mode, 0, 1, den, a, b, ang, flags, out),
+                eng.metal_pass(params, mode, 1, 0, den, a, b, ang, flags, out)]
+        return out, reps
+
+    for w in range(args.warmup):
+        step()
+    wall, kernel, reps = [], [], None
+    for k in range(args.steps):
+        t = time.perf_counter()
+        out, reps = step()
+        wall.append(time.perf_counter() - t)
+        kernel.append(sum(r["kernel_ms"] for r in reps))
+        log("metal step %d: wall %.1f ms, kernels %.1f ms" % (k, wall[-1] * 1e3, kernel[-1]))
+    kernel_ms = float(np.mean(kernel))
+    visited = sum(r["visited_pairs"] for r in reps)
+    data_pairs = sum(r["data_pairs"] for r in reps)
+    model_pairs = sum(r["model_pairs"] for r in reps)
+    flops = (METAL_FLOPS_VISITED * visited + METAL_FLOPS_DATA * data_pairs
+             + METAL_FLOPS_MODEL * model_pairs)
+    achieved = flops / (kernel_ms / 1e3) / 1e12
+    result = {
+        "metric": "metal distortion-matrix in-range pixel pairs/s (1 GPU)",
+        "value": data_pairs / (kernel_ms / 1e3), "unit": "pixel pairs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * float(np.mean(wall)),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "metal_dmat_100k", "scale": args.scale,
+                   "forests": store.num_forests, "neighbour_pairs": pairs.num_pairs,
+                   "kept_pairs": int(len(a)), "reject": args.reject,
+                   "absorbers": "LYA x SiIII(1207), both passes", "bins": "50x50",
+                   "model_bins": "50x50", "visited_pixel_pairs": int(visited),
+                   "data_pixel_pairs": int(data_pairs), "model_pixel_pairs": int(model_pairs),
+                   "device": info["name"],
+                   "timing": "value: in-range pixel pairs of both passes over the two "
+                             "kernels' HIP-event time; ms_per_step includes the pair upload "
+                             "and the 50 MB matrix download of each pass"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_VALU_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_VALU_PEAK_TFLOPS,
+                     "traffic": None, "flops_per_launch": flops / 2, "launches_per_step": 2,
+                     "launch_avg_ms": kernel_ms / 2, "kernel": "picca::metal_kernel",
+                     "kernel_ms": kernel_ms},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        from oracle import cf_oracle as O
+        rng = np.random.default_rng(1)
+        sel = np.sort(rng.choice(len(a), min(len(a), 3000), replace=False))
+        oparams = O.params_from_module(MetalGlobals)
+        ref = metal.outputs(nb, nm)
+        stats = np.zeros(3, dtype=np.int64)
+        off = store.offsets
+        tc = time.time()
+        for k in sel:
+            sa = slice(off[a[k]], off[a[k] + 1])
+            sb = slice(off[b[k]], off[b[k] + 1])
+            for ta, tb in ((lya, si), (si, lya)):
+                O.metal_pair(oparams, mode, den, (store.r_comov[sa], store.dist_m[sa]),
+                             [x[sa] for x in ta], (store.r_comov[sb], store.dist_m[sb]),
+                             [x[sb] for x in tb], ang[k], bool(flags[k]), ref, stats)
+        cpu_s = time.time() - tc
+        mine = metal.outputs(nb, nm)
+        sreps = [eng.metal_pass(params, mode, 0, 1, den, a[sel], b[sel], ang[sel], flags[sel],
+                                mine),
+                 eng.metal_pass(params, mode, 1, 0, den, a[sel], b[sel], ang[sel], flags[sel],
+                                mine)]
+        err = max(float(np.abs(m - r).max() / max(np.abs(r).max(), 1e-300))
+                  for m, r in zip(mine, ref))
+        result["cpu_baseline"] = {
+            "value": stats[0] / cpu_s, "unit": "pixel pairs/s", "cores": 1, "kind": "port",
+            "sample": "%d random kept pairs (seed 1), both passes, %d in-range pixel pairs, "
+                      "FP64 C oracle (scalar restatement of compute_metal_dmat)"
+                      % (len(sel), int(stats[0])), "seconds": cpu_s}
+        result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
+        result["parity_sample"] = {
+            "max_abs_err_over_max": err,
+            "counts_equal": bool(sum(r["data_pairs"] for r in sreps) == stats[0]
+                                 and sum(r["model_pairs"] for r in sreps) == stats[1])}
     print(json.dumps(result))
 
 

@@ -208,3 +208,41 @@ def test_empty_and_invalid_inputs():
         eng.metal_pass(params, 0, 2, 2, 1.0, a, b + store.num_forests, ang, flags, out)
     with pytest.raises(RuntimeError):
         eng.metal_pass(params, 0, 3, 2, 1.0, a, b, ang, flags, out)
+
+
+def compare_metal_files(mine_path, golden_name):
+    mine = cor_cases.FITS(mine_path)
+    golden = cor_cases.FITS(cor_cases.golden_path(golden_name))
+    for key in ("NPALL", "NPUSED"):
+        assert np.array_equal(mine["ATTRI"][key][:], golden["ATTRI"][key][:]), key
+    assert [n.strip() for n in mine["ATTRI"]["ABS_IGM"][:]] == \
+        [n.strip() for n in golden["ATTRI"]["ABS_IGM"][:]]
+    assert mine["MDMAT"].get_colnames() == golden["MDMAT"].get_colnames()
+    result = {c: mine["MDMAT"][c][:] for c in mine["MDMAT"].get_colnames()}
+    cor_cases.compare_to_golden(result, cor_cases.golden_columns(golden_name))
+
+
+METAL_DRIVER_CASES = {
+    "metal_dmat": METAL_CASES["metal_dmat"] + ["--abs-igm", "SiIII(1207)"],
+    "metal_dmat_cross": METAL_CASES["metal_dmat_cross"] + ["--abs-igm", "SiIII(1207)",
+                                                           "--abs-igm2", "SiIII(1207)"],
+}
+
+
+@pytest.mark.parametrize("case", sorted(METAL_DRIVER_CASES))
+def test_metal_dmat_driver_reproduces_golden(case, tmp_path):
+    from picca_amd.bin import picca_metal_dmat
+    out = str(tmp_path / (case + ".fits.gz"))
+    picca_metal_dmat.main(METAL_DRIVER_CASES[case] + ["--out", out])
+    compare_metal_files(out, case)
+
+
+def test_metal_xdmat_driver_reproduces_golden(tmp_path):
+    from picca_amd.bin import picca_metal_xdmat
+    out = str(tmp_path / "metal_xdmat.fits.gz")
+    argv = ["--in-dir", cor_cases.DELTA_LYA, "--drq", cor_cases.CAT, "--abs-igm",
+            "SiIII(1207)", "--rp-min", "-60.0", "--rp-max", "+60.0", "--rt-max", "+60.0",
+            "--np", "30", "--nt", "15", "--rej", "0.99", "--nproc", "1", "--z-evol-obj", "1.",
+            "--in-attributes", cor_cases.ATTRIBUTES, "--out", out]
+    picca_metal_xdmat.main(argv)
+    compare_metal_files(out, "metal_xdmat")
