@@ -18,6 +18,7 @@
  *                      utils.py:12-136 (the Migrad fit of its chi2), batched over forests
  *   picca_metal_*      compute_metal_dmat    py/picca/cf.py:890-1232 and
  *                      py/picca/xcf.py:677-800, one absorber pass over the kept pairs
+ *   picca_cov_subsample compute_cov       py/picca/utils.py:100-127
  *   picca_neighbour_candidates  the healpy query_disc superset of fill_neighs
  *                      py/picca/cf.py:82-135 and py/picca/xcf.py:71-128
  *   picca_cf_params    the module globals    py/picca/cf.py:28-79 (None -> has_* = 0)
@@ -316,6 +317,16 @@ int picca_metal_pass(picca_ctx *ctx, const picca_cf_params *params, int32_t mode
                      const uint8_t *pair_flags, double *weights_dmat, double *dmat,
                      double *r_par_eff, double *r_trans_eff, double *z_eff,
                      double *weight_eff, picca_metal_report *report);
+
+/*
+ * Covariance of the per-healpix correlation rows by subsampling (compute_cov of
+ * py/picca/utils.py:100-127, called by picca_export, py/picca/bin/picca_export.py:276):
+ * xi and weights are [num_samples][num_bins] row-major (the COR rows' DA and WE),
+ * cov receives [num_bins][num_bins].  kernel_ms (may be NULL) receives the device time.
+ */
+int picca_cov_subsample(picca_ctx *ctx, int64_t num_samples, int64_t num_bins,
+                        const double *xi, const double *weights, double *cov,
+                        double *kernel_ms);
 
 #ifdef __cplusplus
 }

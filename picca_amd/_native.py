@@ -112,6 +112,7 @@ _PROTOTYPES = {
     "picca_xi_download": [_P, _P, _P, _P, _P, _P, _P],
     "picca_dmat_run": [_P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "picca_diag_copy": [_P, _I64, _P],
+    "picca_cov_subsample": [_P, _I64, _I64, _P, _P, _P, _P],
     "picca_metal_set_absorber": [_P, _I32, _P, _P, _P, _P, _P],
     "picca_metal_pass": [_P, _P, _I32, _I32, _I32, _F64, _I64, _P, _P, _P, _P, _P, _P, _P, _P,
                          _P, _P, _P],

@@ -158,6 +158,9 @@ struct picca_ctx {
     DeviceBuffer<int32_t> m_task_pair;
     DeviceBuffer<unsigned long long> m_stats;
 
+    // subsampling covariance: rows, centred weighted rows, per-bin weight sums, matrix
+    DeviceBuffer<double> cv_xi, cv_w, cv_y, cv_s, cv_cov;
+
     // continuum fit: resident spectra (forest pixels) and per-forest results
     int64_t cont_forests = 0;
     int64_t cont_pixels = 0;

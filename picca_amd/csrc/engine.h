@@ -276,6 +276,14 @@ hipError_t launch_metal(const MetalConst &C, const int64_t *off, const uint8_t *
 size_t metal_lds_bytes(const MetalConst &C);
 
 // ---------------------------------------------------------------------------------------
+// subsampling covariance
+// ---------------------------------------------------------------------------------------
+
+// S[num_bins] and Y[num_samples][num_bins] are workspace; cov is [num_bins][num_bins]
+hipError_t launch_cov(const double *xi, const double *w, int64_t num_samples, int num_bins,
+                      double *S, double *Y, double *cov, hipStream_t stream);
+
+// ---------------------------------------------------------------------------------------
 // continuum fit
 // ---------------------------------------------------------------------------------------
 
