@@ -37,7 +37,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 METRIC = "weighted pixel pairs/sec (1 GPU) vs numba host; xi err vs FP64; fits/s"
-FP64_VALU_PEAK_TFLOPS = 78.6   # MI355X FP64 vector, published
+FP64_VALU_PEAK_TFLOPS = 78.6   # MI355X FP64 vector and dense FP64 matrix, published
 HBM_PEAK_GBS = 8000.0
 # FP64 operations per pair in the reference's loop (r_par sub+mul, r_trans add+mul for
 # every evaluated pair; z 2, bins 5, products 3, accumulations 5 adds + 3 muls more for
@@ -135,12 +135,13 @@ def main():
                              "counter-derived figure in profiles/ for this workload)")
     parser.add_argument("--mode", default="xi",
                         choices=("xi", "dmat", "continuum", "precision", "xcf", "xdmat",
-                                 "metal"),
+                                 "metal", "cov"),
                         help="xi: the pair kernel (default); dmat: the distortion matrix; "
                              "continuum: plain vs learned-seed continuum fits; precision: "
                              "xi error and edge pairs on the heavy-tailed survey; xcf / xdmat: "
                              "forest x quasar correlation / distortion matrix; metal: the "
-                             "LYA x SiIII(1207) metal distortion matrix")
+                             "LYA x SiIII(1207) metal distortion matrix; cov: the "
+                             "subsampling covariance of picca_export")
     parser.add_argument("--reject", type=float, default=0.99,
                         help="dmat: fraction of neighbour pairs rejected")
     parser.add_argument("--spectra", type=int, default=200000,
@@ -161,6 +162,8 @@ def main():
         return bench_xcf(args)
     if args.mode == "metal":
         return bench_metal(args)
+    if args.mode == "cov":
+        return bench_cov(args)
     return bench_xi(args)
 
 
@@ -513,6 +516,73 @@ def bench_metal(args):
             "max_abs_err_over_max": err,
             "counts_equal": bool(sum(r["data_pairs"] for r in sreps) == stats[0]
                                  and sum(r["model_pairs"] for r in sreps) == stats[1])}
+    print(json.dumps(result))
+
+
+def bench_cov(args):
+    """Subsampling covariance of picca_export (compute_cov) on COR rows of the 100k-forest
+    survey's shape: 1196 healpix rows (nside 64) x 2500 bins (50 x 50), synthetic xi and
+    weights; the timed work is the whole computation on inputs resident in HBM."""
+    from picca_amd.engine import device_info, get_engine
+
+    rows = max(1, int(round(1196 * args.scale)))
+    nb = 2500
+    rng = np.random.default_rng(args.seed)
+    xi = rng.normal(0.0, 1e-2, size=(rows, nb))
+    we = rng.uniform(1e3, 1e5, size=(rows, nb))
+    eng = get_engine(0)
+    info = device_info(0)
+    for w in range(args.warmup):
+        eng.cov_subsample(xi, we)
+    total, syrk, cov = [], [], None
+    for k in range(max(args.steps, 1) * 10):
+        cov, ms, ms_syrk = eng.cov_subsample(xi, we)
+        total.append(ms)
+        syrk.append(ms_syrk)
+    total_ms = float(np.mean(total))
+    syrk_ms = float(np.mean(syrk))
+    flops = float(nb) * (nb + 1) * rows      # upper triangle of Y^T Y, multiply + add
+    achieved = flops / (syrk_ms / 1e3) / 1e12
+    result = {
+        "metric": "subsampling covariances/s (1 GPU)",
+        "value": 1e3 / total_ms, "unit": "covariances/s", "n_gpus": 1,
+        "steps": len(total), "warmup": args.warmup, "ms_per_step": total_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "cov_100k", "rows": rows, "bins": nb, "device": info["name"],
+                   "timing": "HIP events around the two kernels (weighted centring, then "
+                             "Y^T Y on the FP64 matrix cores); inputs uploaded before"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_VALU_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_VALU_PEAK_TFLOPS,
+                     "traffic": None, "flops_per_launch": flops,
+                     "kernel": "picca::cov_syrk_kernel", "kernel_ms": syrk_ms},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        from oracle import cov_oracle as C
+        try:
+            from threadpoolctl import threadpool_info
+            cores = max([p.get("num_threads", 1) for p in threadpool_info()] or [1])
+        except ImportError:
+            cores = os.cpu_count()
+        reps = 0
+        tc = time.time()
+        while True:
+            ref = C.compute_cov(xi, we)
+            reps += 1
+            if time.time() - tc > 10.0 or reps >= 50:
+                break
+        cpu_s = (time.time() - tc) / reps
+        scale = np.abs(ref).max()
+        result["cpu_baseline"] = {
+            "value": 1.0 / cpu_s, "unit": "covariances/s", "cores": int(cores),
+            "kind": "port", "sample": "the same %d x %d rows, numpy (BLAS) restatement of "
+                                      "compute_cov, %d repetitions" % (rows, nb, reps),
+            "seconds": cpu_s}
+        result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
+        result["parity_sample"] = {
+            "max_abs_err_over_max": float(np.abs(cov - ref).max() / scale),
+            "symmetric": bool(np.array_equal(cov, cov.T))}
     print(json.dumps(result))
 
 

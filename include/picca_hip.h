@@ -322,7 +322,8 @@ int picca_metal_pass(picca_ctx *ctx, const picca_cf_params *params, int32_t mode
  * Covariance of the per-healpix correlation rows by subsampling (compute_cov of
  * py/picca/utils.py:100-127, called by picca_export, py/picca/bin/picca_export.py:276):
  * xi and weights are [num_samples][num_bins] row-major (the COR rows' DA and WE),
- * cov receives [num_bins][num_bins].  kernel_ms (may be NULL) receives the device time.
+ * cov receives [num_bins][num_bins].  kernel_ms (may be NULL) receives two device times
+ * in ms: [0] the whole computation, [1] the matrix-core product Y^T Y alone.
  */
 int picca_cov_subsample(picca_ctx *ctx, int64_t num_samples, int64_t num_bins,
                         const double *xi, const double *weights, double *cov,

@@ -127,6 +127,7 @@ int picca_create(int32_t device, picca_ctx **out) {
     if (err == hipSuccess) err = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (err == hipSuccess) err = hipEventCreate(&ctx->ev_start);
     if (err == hipSuccess) err = hipEventCreate(&ctx->ev_stop);
+    if (err == hipSuccess) err = hipEventCreate(&ctx->ev_mid);
     if (err != hipSuccess) {
         picca_destroy(ctx);
         return fail(PICCA_E_HIP, std::string("picca_create: ") + hipGetErrorString(err));
@@ -143,6 +144,7 @@ int picca_destroy(picca_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+    if (ctx->ev_mid) (void)hipEventDestroy(ctx->ev_mid);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PICCA_OK;

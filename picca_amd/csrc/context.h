@@ -100,7 +100,7 @@ using picca_host::parallel_for;
 struct picca_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_mid = nullptr;
     int num_cus = 0;
     int64_t lds_limit = 0;
 
@@ -158,7 +158,7 @@ struct picca_ctx {
     DeviceBuffer<int32_t> m_task_pair;
     DeviceBuffer<unsigned long long> m_stats;
 
-    // subsampling covariance: rows, centred weighted rows, per-bin weight sums, matrix
+    // subsampling covariance: rows, weights, per-bin means and weight sums, matrix
     DeviceBuffer<double> cv_xi, cv_w, cv_y, cv_s, cv_cov;
 
     // continuum fit: resident spectra (forest pixels) and per-forest results

@@ -279,9 +279,12 @@ size_t metal_lds_bytes(const MetalConst &C);
 // subsampling covariance
 // ---------------------------------------------------------------------------------------
 
-// S[num_bins] and Y[num_samples][num_bins] are workspace; cov is [num_bins][num_bins]
-hipError_t launch_cov(const double *xi, const double *w, int64_t num_samples, int num_bins,
-                      double *S, double *Y, double *cov, hipStream_t stream);
+// S[num_bins] and mean[num_bins] are workspace; cov is [num_bins][num_bins]
+hipError_t launch_cov_prep(const double *xi, const double *w, int64_t num_samples,
+                           int num_bins, double *S, double *mean, hipStream_t stream);
+hipError_t launch_cov_syrk(const double *xi, const double *w, const double *S,
+                           const double *mean, int64_t num_samples, int num_bins, double *cov,
+                           hipStream_t stream);
 
 // ---------------------------------------------------------------------------------------
 // continuum fit

@@ -241,17 +241,17 @@ class Engine:
     # -- subsampling covariance --------------------------------------------------
     def cov_subsample(self, xi, weights):
         """compute_cov of the rows of xi [num_samples, num_bins]; returns
-        (covariance [num_bins, num_bins], kernel_ms)."""
+        (covariance [num_bins, num_bins], total kernel ms, matrix-product kernel ms)."""
         xi = np.ascontiguousarray(xi, dtype=np.float64)
         weights = np.ascontiguousarray(weights, dtype=np.float64)
         if xi.ndim != 2 or xi.shape != weights.shape:
             raise ValueError("xi and weights must be 2-d arrays of one shape")
         ns, nb = xi.shape
         cov = np.zeros((nb, nb))
-        ms = ctypes.c_double(0.0)
+        ms = np.zeros(2)
         check(self._lib.picca_cov_subsample(self._ctx, ns, nb, ptr(xi), ptr(weights), ptr(cov),
-                                            ctypes.byref(ms)), "picca_cov_subsample")
-        return cov, ms.value
+                                            ptr(ms)), "picca_cov_subsample")
+        return cov, float(ms[0]), float(ms[1])
 
     # -- metal distortion matrix -------------------------------------------------
     def metal_set_absorber(self, slot, z, r_comov, dist_m, weights, zfac):

@@ -25,17 +25,17 @@ def test_cov_matches_oracle(ns, nb):
     we[rng.random(size=we.shape) < 0.05] = 0.0
     if nb > 3:
         we[:, 3] = 0.0           # a bin without weight: S = 0, left unnormalised
-    mine, ms = get_engine(0).cov_subsample(xi, we)
+    mine, ms, ms_syrk = get_engine(0).cov_subsample(xi, we)
     ref = C.compute_cov(xi, we)
     scale = np.abs(ref).max()
     np.testing.assert_allclose(mine, ref, rtol=1e-10, atol=1e-12 * scale)
     assert np.array_equal(mine, mine.T)
-    assert ms >= 0
+    assert ms >= ms_syrk >= 0
 
 
 def test_cov_empty_and_invalid():
     eng = get_engine(0)
-    cov, _ = eng.cov_subsample(np.zeros((0, 4)), np.zeros((0, 4)))
+    cov, _, _ = eng.cov_subsample(np.zeros((0, 4)), np.zeros((0, 4)))
     assert cov.shape == (4, 4) and not cov.any()
     with pytest.raises(ValueError):
         eng.cov_subsample(np.full((2, 3), np.nan), np.ones((2, 3)))
