@@ -19,6 +19,7 @@
  *   picca_metal_*      compute_metal_dmat    py/picca/cf.py:890-1232 and
  *                      py/picca/xcf.py:677-800, one absorber pass over the kept pairs
  *   picca_cov_subsample compute_cov       py/picca/utils.py:100-127
+ *   picca_wick_t123    compute_wickT123_pairs py/picca/cf.py:1486-1602 over a pair list
  *   picca_neighbour_candidates  the healpy query_disc superset of fill_neighs
  *                      py/picca/cf.py:82-135 and py/picca/xcf.py:71-128
  *   picca_cf_params    the module globals    py/picca/cf.py:28-79 (None -> has_* = 0)
@@ -328,6 +329,34 @@ int picca_metal_pass(picca_ctx *ctx, const picca_cf_params *params, int32_t mode
 int picca_cov_subsample(picca_ctx *ctx, int64_t num_samples, int64_t num_bins,
                         const double *xi, const double *weights, double *cov,
                         double *kernel_ms);
+
+/* Counters of one Wick call (no counterpart in picca). */
+typedef struct picca_wick_report {
+    int64_t in_range_pairs;  /* listed pixel pairs (sum of num_pairs_wick)           */
+    int64_t products;        /* (a, b) pixel-pair products evaluated                 */
+    int64_t overflow_pairs;  /* in-range pairs whose bin rounds up to np (skipped;
+                                numba writes them out of bounds)                       */
+    int64_t batches;         /* batches the pair list was split into                   */
+    double kernel_ms;        /* device time of the kernels (HIP events)                */
+} picca_wick_report;
+
+/*
+ * Wick covariance terms T1, T2, T3 (compute_wickT123_pairs, py/picca/cf.py:1486-1602) over
+ * an explicit list of forest pairs of the resident store (compute_wick_terms,
+ * py/picca/cf.py:1328-1424, draws one uniform per delta of a healpix and takes every
+ * neighbour of a kept delta: the caller makes those draws).  Uses the store's r_comov and
+ * weights; zev1 / zev2 [store pixels] are ((1+z)/(1+z_ref))^(alpha-1) and ^(alpha2-1);
+ * wx + wx_offsets[f] is forest f's n x n weighted 1-D correlation
+ * w_k w_l xi_1d(|log_lambda_k - log_lambda_l|) sqrt(v_k v_l) (py/picca/cf.py:1392-1398),
+ * needed for every forest of the pair list.  Outputs (accumulated): weights_wick[nb],
+ * num_pairs_wick[nb], t1, t2, t3 [nb][nb].  report may be NULL.
+ */
+int picca_wick_t123(picca_ctx *ctx, const picca_cf_params *params, int64_t num_pairs,
+                    const int32_t *pair_a, const int32_t *pair_b, const double *pair_ang,
+                    const double *zev1, const double *zev2, const int64_t *wx_offsets,
+                    int64_t wx_size, const double *wx, double *weights_wick,
+                    int64_t *num_pairs_wick, double *t1, double *t2, double *t3,
+                    picca_wick_report *report);
 
 #ifdef __cplusplus
 }

@@ -18,6 +18,8 @@
  *                        py/picca/xcf.py:427-658 (one delta with its kept quasars)
  *   oracle_metal_pair <- one pass of compute_metal_dmat for one pair,
  *                        py/picca/cf.py:890-1232 and py/picca/xcf.py:677-800
+ *   oracle_wick_t123  <- compute_wickT123_pairs over a list of pairs,
+ *                        py/picca/cf.py:1486-1602
  *
  * Loop order, branch order and every floating-point operation follow the reference
  * (divide-then-multiply bin assignment, cos/sin of ang/2 from libm, sequential sums
@@ -828,5 +830,124 @@ void oracle_metal_pair(const oracle_params *p, int32_t mode, double zw_den,
             weight_eff[model] += wz;
             stats[1] += 1;
         }
+    }
+}
+
+
+/*
+ * Wick terms T1, T2, T3 of the auto-correlation covariance for a list of forest pairs
+ * (compute_wickT123_pairs, py/picca/cf.py:1486-1602, called per pair by
+ * compute_wick_terms, :1380-1424).  Forest f has pixels offsets[f] .. offsets[f+1]-1 with
+ * r_comov, weights and the evolution factors zev1 = ((1+z)/(1+z_ref))^(alpha-1),
+ * zev2 = ((1+z)/(1+z_ref))^(alpha2-1); wx + wx_offsets[f] is its n x n weighted 1-D
+ * correlation (w_k w_l xi_1d(|log_lambda_k - log_lambda_l|) sqrt(v_k v_l)).
+ * r_trans is (r1 + r2) sin(ang/2) with the comoving distances, as the reference.
+ * The pairs are split over OpenMP threads with private accumulators, summed at the end.
+ */
+static void wick_pair(const oracle_params *p, const double *r1, const double *w1,
+                      const double *za1, int64_t n1, const double *r2, const double *w2,
+                      const double *zb2, int64_t n2, double ang, const double *wx1,
+                      const double *wx2, double *weights_wick, int64_t *num_pairs_wick,
+                      double *t1, double *t2, double *t3, int64_t nb) {
+    const double c = cos(ang / 2), s = sin(ang / 2);
+    const double span = p->r_par_max - p->r_par_min;
+    int64_t count = 0;
+    for (int64_t j = 0; j < n2; ++j)
+        for (int64_t i = 0; i < n1; ++i) {
+            double r_par = (r1[i] - r2[j]) * c;
+            if (!p->x_correlation) r_par = fabs(r_par);
+            const double r_trans = (r1[i] + r2[j]) * s;
+            if (r_par < p->r_par_max && r_trans < p->r_trans_max && r_par >= p->r_par_min)
+                ++count;
+        }
+    if (count == 0) return;
+    int64_t *pi = (int64_t *)malloc(sizeof(int64_t) * (size_t)count);
+    int64_t *pj = (int64_t *)malloc(sizeof(int64_t) * (size_t)count);
+    int64_t *pb = (int64_t *)malloc(sizeof(int64_t) * (size_t)count);
+    int64_t k = 0;
+    for (int64_t j = 0; j < n2; ++j)
+        for (int64_t i = 0; i < n1; ++i) {
+            double r_par = (r1[i] - r2[j]) * c;
+            if (!p->x_correlation) r_par = fabs(r_par);
+            const double r_trans = (r1[i] + r2[j]) * s;
+            if (!(r_par < p->r_par_max && r_trans < p->r_trans_max && r_par >= p->r_par_min))
+                continue;
+            const int64_t bp = (int64_t)((r_par - p->r_par_min) / span * p->num_bins_r_par);
+            const int64_t bt = (int64_t)(r_trans / p->r_trans_max * p->num_bins_r_trans);
+            pi[k] = i;
+            pj[k] = j;
+            pb[k] = bt + (int64_t)p->num_bins_r_trans * bp;
+            ++k;
+        }
+    for (int64_t a = 0; a < count; ++a) {
+        const int64_t p1 = pb[a], i1 = pi[a], j1 = pj[a];
+        if (p1 < 0 || p1 >= nb) continue;   /* rounding up to np: numba writes out of bounds */
+        const double w12 = w1[i1] * w2[j1];
+        weights_wick[p1] += w12;
+        num_pairs_wick[p1] += 1;
+        t1[p1 * nb + p1] += w12 * (za1[i1] * zb2[j1]);
+        for (int64_t b = a + 1; b < count; ++b) {
+            const int64_t p2 = pb[b], i2 = pi[b], j2 = pj[b];
+            if (p2 < 0 || p2 >= nb) continue;
+            double prod;
+            if (i1 == i2) {
+                prod = wx2[j1 * n2 + j2] * w1[i1] * za1[i1];
+                t2[p1 * nb + p2] += prod;
+                t2[p2 * nb + p1] += prod;
+            } else if (j1 == j2) {
+                prod = wx1[i1 * n1 + i2] * w2[j2] * zb2[j1];
+                t2[p1 * nb + p2] += prod;
+                t2[p2 * nb + p1] += prod;
+            } else {
+                prod = wx1[i1 * n1 + i2] * wx2[j1 * n2 + j2];
+                t3[p1 * nb + p2] += prod;
+                t3[p2 * nb + p1] += prod;
+            }
+        }
+    }
+    free(pi);
+    free(pj);
+    free(pb);
+}
+
+void oracle_wick_t123(const oracle_params *p, int64_t num_pairs, const int32_t *pair_a,
+                      const int32_t *pair_b, const double *pair_ang, const int64_t *offsets,
+                      const double *r_comov, const double *weights, const double *zev1,
+                      const double *zev2, const int64_t *wx_offsets, const double *wx,
+                      double *weights_wick, int64_t *num_pairs_wick, double *t1, double *t2,
+                      double *t3) {
+    const int64_t nb = (int64_t)p->num_bins_r_par * p->num_bins_r_trans;
+#pragma omp parallel
+    {
+        double *lw = (double *)calloc((size_t)nb, sizeof(double));
+        int64_t *ln = (int64_t *)calloc((size_t)nb, sizeof(int64_t));
+        double *l1 = (double *)calloc((size_t)(nb * nb), sizeof(double));
+        double *l2 = (double *)calloc((size_t)(nb * nb), sizeof(double));
+        double *l3 = (double *)calloc((size_t)(nb * nb), sizeof(double));
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t k = 0; k < num_pairs; ++k) {
+            const int64_t a = pair_a[k], b = pair_b[k];
+            const int64_t oa = offsets[a], ob = offsets[b];
+            wick_pair(p, r_comov + oa, weights + oa, zev1 + oa, offsets[a + 1] - oa,
+                      r_comov + ob, weights + ob, zev2 + ob, offsets[b + 1] - ob, pair_ang[k],
+                      wx + wx_offsets[a], wx + wx_offsets[b], lw, ln, l1, l2, l3, nb);
+        }
+#pragma omp critical
+        {
+            for (int64_t q = 0; q < nb; ++q) {
+                weights_wick[q] += lw[q];
+                num_pairs_wick[q] += ln[q];
+            }
+            for (int64_t q = 0; q < nb * nb; ++q) {
+                t1[q] += l1[q];
+                t2[q] += l2[q];
+                t3[q] += l3[q];
+            }
+        }
+        free(lw);
+        free(ln);
+        free(l1);
+        free(l2);
+        free(l3);
     }
 }

@@ -18,6 +18,9 @@ module, and only as the checker; the engine never imports it.
 * ``compute_metal_dmat`` / ``compute_metal_xdmat`` restate ``compute_metal_dmat`` of
   ``py/picca/cf.py:890-1232`` and ``py/picca/xcf.py:677-800`` with the C pass
   ``oracle_metal_pair``.
+* ``read_cf1d`` / ``compute_wick_terms`` restate the 1-D inputs of picca_wick
+  (``py/picca/bin/picca_wick.py:392-419``) and ``compute_wick_terms``
+  (``py/picca/cf.py:1328-1483``, diagrams 1-3) with the C pair loop ``oracle_wick_t123``.
 """
 import ctypes
 import os
@@ -151,6 +154,9 @@ def lib():
                                            + [P] * 7 + [ctypes.c_int64, ctypes.c_double,
                                                         ctypes.c_int32] + [P] * 7)
         _lib.oracle_metal_pair.restype = None
+        _lib.oracle_wick_t123.argtypes = ([ctypes.POINTER(OracleParams), ctypes.c_int64]
+                                          + [P] * 15)
+        _lib.oracle_wick_t123.restype = None
     return _lib
 
 
@@ -596,3 +602,87 @@ def compute_metal_xdmat(params, data, healpixs, neighbours, reject, cosmo, abs_i
     out[1] = out[1].reshape(nb, nm)
     return (out[0], out[1], out[2], out[3], out[4], out[5], num_pairs,
             int(num_pairs_used)), stats
+
+
+# ---------------------------------------------------------------------------
+# Wick covariance, diagrams T1-T3 (py/picca/cf.py:1328-1602)
+# ---------------------------------------------------------------------------
+
+def read_cf1d(path):
+    """The nearest-neighbour interpolations of the 1-D variance and correlation of a
+    picca_cf1d file (py/picca/bin/picca_wick.py:392-419)."""
+    from scipy.interpolate import interp1d
+
+    from picca_amd.fitsio_lite import FITS
+    hdu = FITS(path)[1]
+    header = hdu.read_header()
+    log_lambda_min = header["LLMIN"]
+    delta_log_lambda = header["DLL"]
+    nv1d = hdu["nv1d"][:]
+    v1d = hdu["v1d"][:]
+    log_lambda = log_lambda_min + delta_log_lambda * np.arange(len(v1d))
+    variance = interp1d(log_lambda[nv1d > 0], v1d[nv1d > 0], kind="nearest",
+                        fill_value="extrapolate")
+    nb1d = hdu["nb1d"][:]
+    c1d = hdu["c1d"][:]
+    xi = interp1d((log_lambda - log_lambda_min)[nb1d > 0], c1d[nb1d > 0], kind="nearest",
+                  fill_value="extrapolate")
+    return variance, xi
+
+
+def weighted_xi_1d(delta, variance, xi):
+    """(w w^T) xi_1d(|log_lambda - log_lambda^T|) sqrt(v v^T) of one delta
+    (py/picca/cf.py:1392-1398)."""
+    v = variance(delta.log_lambda)
+    w = delta.weights
+    return (w * w[:, None]) * xi(abs(delta.log_lambda - delta.log_lambda[:, None])) * \
+        np.sqrt(v * v[:, None])
+
+
+def compute_wick_terms(params, data, healpixs, neighbours, reject, variance, xi):
+    """compute_wick_terms of the auto-correlation (diagrams 1-3) with the legacy global
+    numpy stream: one draw per delta of each healpix, all neighbours of a kept delta.
+    Returns (weights_wick, num_pairs_wick, num_pairs, num_pairs_used, t1, t2, t3)."""
+    nb = params.num_bins_r_par * params.num_bins_r_trans
+    pairs = []
+    num_pairs = 0
+    num_pairs_used = 0
+    for hp in healpixs:
+        w = np.random.rand(len(data[hp])) > reject
+        num_pairs += len(data[hp])
+        num_pairs_used += w.sum()
+        if w.sum() == 0:
+            continue
+        for d1, keep in zip(data[hp], w):
+            if not keep:
+                continue
+            for d2 in neighbours[id(d1)]:
+                pairs.append((d1, d2, angle(d1, d2)))
+    deltas = []
+    index = {}
+    for d1, d2, _ in pairs:
+        for d in (d1, d2):
+            if id(d) not in index:
+                index[id(d)] = len(deltas)
+                deltas.append(d)
+    lengths = np.array([len(d.r_comov) for d in deltas], dtype=np.int64)
+    offsets = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
+    cat = lambda f: np.concatenate([f(d) for d in deltas]) if deltas else np.zeros(0)  # noqa
+    r = _f(cat(lambda d: d.r_comov))
+    wt = _f(cat(lambda d: d.weights))
+    zev1 = _f(cat(lambda d: ((1 + d.z) / (1 + params.z_ref)) ** (params.alpha - 1)))
+    zev2 = _f(cat(lambda d: ((1 + d.z) / (1 + params.z_ref)) ** (params.alpha2 - 1)))
+    wx = [weighted_xi_1d(d, variance, xi).ravel() for d in deltas]
+    wx_off = np.concatenate([[0], np.cumsum([len(m) for m in wx])]).astype(np.int64)
+    wx = _f(np.concatenate(wx) if wx else np.zeros(0))
+    pa = np.array([index[id(p[0])] for p in pairs], dtype=np.int32)
+    pb = np.array([index[id(p[1])] for p in pairs], dtype=np.int32)
+    ang = _f([p[2] for p in pairs])
+    weights_wick = np.zeros(nb)
+    num_pairs_wick = np.zeros(nb, dtype=np.int64)
+    t = [np.zeros(nb * nb) for _ in range(3)]
+    lib().oracle_wick_t123(ctypes.byref(params), len(pairs), _p(pa), _p(pb), _p(ang),
+                           _p(offsets), _p(r), _p(wt), _p(zev1), _p(zev2), _p(wx_off), _p(wx),
+                           _p(weights_wick), _p(num_pairs_wick), *[_p(x) for x in t])
+    return (weights_wick, num_pairs_wick, num_pairs, int(num_pairs_used),
+            t[0].reshape(nb, nb), t[1].reshape(nb, nb), t[2].reshape(nb, nb))

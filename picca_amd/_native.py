@@ -74,6 +74,20 @@ class XiReport(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class WickReport(ctypes.Structure):
+    """``picca_wick_report``."""
+    _fields_ = [
+        ("in_range_pairs", ctypes.c_int64),
+        ("products", ctypes.c_int64),
+        ("overflow_pairs", ctypes.c_int64),
+        ("batches", ctypes.c_int64),
+        ("kernel_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class MetalReport(ctypes.Structure):
     """``picca_metal_report``."""
     _fields_ = [
@@ -113,6 +127,7 @@ _PROTOTYPES = {
     "picca_dmat_run": [_P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "picca_diag_copy": [_P, _I64, _P],
     "picca_cov_subsample": [_P, _I64, _I64, _P, _P, _P, _P],
+    "picca_wick_t123": [_P, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P],
     "picca_metal_set_absorber": [_P, _I32, _P, _P, _P, _P, _P],
     "picca_metal_pass": [_P, _P, _I32, _I32, _I32, _F64, _I64, _P, _P, _P, _P, _P, _P, _P, _P,
                          _P, _P, _P],

@@ -11,8 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBRARY = os.path.join(HERE, "libpicca_hip.so")
 SOURCES = ["api.hip", "cont_api.hip", "mlp_api.hip", "metal_api.hip", "cov_api.hip",
-           "xi_kernel.hip", "xcf_kernel.hip", "dmat_kernel.hip", "metal_kernel.hip",
-           "cov_kernel.hip",
+           "wick_api.hip", "xi_kernel.hip", "xcf_kernel.hip", "dmat_kernel.hip",
+           "metal_kernel.hip", "cov_kernel.hip", "wick_kernel.hip",
            "cont_kernel.hip", "mlp_kernel.hip",
            "neighbours.cpp"]
 

@@ -59,6 +59,13 @@ data2 = None
 
 cosmo = None
 
+# Wick covariance inputs (set by picca_wick): nearest-neighbour interpolations of the 1-D
+# correlation and variance per delta field name, the 3-D correlations for diagrams 4-5
+xi_1d = {}
+get_variance_1d = {}
+xi_wick = {}
+max_diagram = None
+
 reject = None
 lock = None
 x_correlation = False
@@ -474,6 +481,79 @@ def compute_metal_dmat(healpixs, abs_igm1="LYA", abs_igm2="SiIII(1207)"):
     last_report = {"passes": reports}
     return (out[0], out[1].reshape(nb, nm), out[2], out[3], out[4], out[5], num_pairs,
             num_pairs_used)
+
+
+# ---------------------------------------------------------------------------
+# Wick covariance
+# ---------------------------------------------------------------------------
+
+def _weighted_xi_1d(delta):
+    """(w w^T) xi_1d(|log_lambda - log_lambda^T|) sqrt(v v^T) of one delta, with the
+    interpolations of its field (``py/picca/cf.py:1392-1398``)."""
+    variance = get_variance_1d[delta.fname](delta.log_lambda)
+    weights = delta.weights
+    return (weights * weights[:, None]) * xi_1d[delta.fname](
+        abs(delta.log_lambda - delta.log_lambda[:, None])) * np.sqrt(variance * variance[:, None])
+
+
+def compute_wick_terms(healpixs):
+    """Wick expansion of the covariance of the auto-correlation (``py/picca/cf.py:1328-1483``)
+    for diagrams 1-3: one draw of numpy's global legacy stream per delta of each healpix,
+    every neighbour of a kept delta, T1-T3 of all pairs in one GPU computation.  Returns
+    (weights_wick, num_pairs_wick, num_pairs, num_pairs_used, t1, t2, t3, t4, t5, t6);
+    t4-t6 are zero (diagrams 4-5 need max_diagram > 3, which this engine refuses)."""
+    global last_report
+    if max_diagram is not None and max_diagram > 3:
+        raise NotImplementedError("diagrams 4 and 5 (max_diagram > 3) are not part of this "
+                                  "engine")
+    cat = _catalogue()
+    nb = num_bins_r_par * num_bins_r_trans
+    pair_a, pair_b, pair_ang = [], [], []
+    num_pairs = 0
+    num_pairs_used = 0
+    for healpix in healpixs:
+        w = np.random.rand(len(data[healpix])) > reject
+        num_pairs += len(data[healpix])
+        num_pairs_used += w.sum()
+        if w.sum() == 0:
+            continue
+        for delta1 in [delta for index, delta in enumerate(data[healpix]) if w[index]]:
+            _tick()
+            neighbours = delta1.neighbours
+            if neighbours is None or len(neighbours) == 0:
+                continue
+            ang = angle_between(delta1.x_cart, delta1.y_cart, delta1.z_cart, delta1.ra,
+                                delta1.dec, delta1.cos_dec,
+                                np.array([d.x_cart for d in neighbours]),
+                                np.array([d.y_cart for d in neighbours]),
+                                np.array([d.z_cart for d in neighbours]),
+                                np.array([d.ra for d in neighbours]),
+                                np.array([d.dec for d in neighbours]))
+            pair_a.append(np.full(len(neighbours), cat.index_of(delta1), dtype=np.int64))
+            pair_b.append(np.array([cat.index_of(d) for d in neighbours], dtype=np.int64))
+            pair_ang.append(np.asarray(ang, dtype=np.float64))
+    pair_a, pair_b, pair_ang = [np.concatenate(x) if x else np.zeros(0) for x in
+                                (pair_a, pair_b, pair_ang)]
+    store = cat.store
+    wx_offsets = np.full(store.num_forests, -1, dtype=np.int64)
+    mats = []
+    used = 0
+    for f in np.unique(np.concatenate([pair_a, pair_b]).astype(np.int64)):
+        m = _weighted_xi_1d(cat.deltas[f]).ravel()
+        wx_offsets[f] = used
+        used += len(m)
+        mats.append(m)
+    wx = np.concatenate(mats) if mats else np.zeros(0)
+    zev1 = ((1 + store.z) / (1 + z_ref)) ** (alpha - 1)
+    zev2 = ((1 + store.z) / (1 + z_ref)) ** (alpha2 - 1)
+    eng = _engine.get_engine(device)
+    eng.set_forests(store, key=cat.key)
+    weights_wick, num_pairs_wick, t1, t2, t3, report = eng.wick_t123(
+        current_params(), pair_a, pair_b, pair_ang, zev1, zev2, wx_offsets, wx)
+    last_report = report
+    zero = np.zeros((nb, nb))
+    return (weights_wick, num_pairs_wick, num_pairs, num_pairs_used, t1, t2, t3, zero,
+            zero.copy(), zero.copy())
 
 
 def compute_dmat_forest_pairs_fast(log_lambda1, log_lambda2, r_comov1, r_comov2, dist_m1,

@@ -161,6 +161,12 @@ struct picca_ctx {
     // subsampling covariance: rows, weights, per-bin means and weight sums, matrix
     DeviceBuffer<double> cv_xi, cv_w, cv_y, cv_s, cv_cov;
 
+    // Wick terms: evolution factors, weighted 1-D correlations, per-batch lists and tasks
+    DeviceBuffer<double> wk_zev1, wk_zev2, wk_wx;
+    DeviceBuffer<int64_t> wk_wxoff, wk_base, wk_list;
+    DeviceBuffer<int32_t> wk_count, wk_task;
+    DeviceBuffer<unsigned long long> wk_nb, wk_stats, wk_ntasks;
+
     // continuum fit: resident spectra (forest pixels) and per-forest results
     int64_t cont_forests = 0;
     int64_t cont_pixels = 0;

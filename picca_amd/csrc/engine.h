@@ -287,6 +287,67 @@ hipError_t launch_cov_syrk(const double *xi, const double *w, const double *S,
                            hipStream_t stream);
 
 // ---------------------------------------------------------------------------------------
+// Wick covariance terms T1-T3
+// ---------------------------------------------------------------------------------------
+
+constexpr int WICK_BLOCK = 256;
+constexpr int WICK_STAGE = 1024;   // pixel pairs a staged in LDS at a time
+
+enum : int {
+    WICK_STAT_IN_RANGE = 0,    // listed pixel pairs
+    WICK_STAT_PRODUCTS = 1,    // (a, b) products evaluated
+    WICK_STAT_OVERFLOW = 2,    // in-range pairs whose bin rounds up to np (skipped)
+    WICK_NSTATS = 3,
+};
+
+struct WickConst {
+    int32_t np, nt, nb;
+    int32_t xcorr;
+    double rpmin, rpmax, rtmax, span;
+};
+
+struct WickInputs {
+    const int64_t *__restrict__ off;     // store offsets
+    const double *__restrict__ r;        // r_comov
+    const double *__restrict__ w;        // weights
+    const double *__restrict__ zev1;     // ((1+z)/(1+z_ref))^(alpha-1)
+    const double *__restrict__ zev2;     // ((1+z)/(1+z_ref))^(alpha2-1)
+    const double *__restrict__ wx;       // weighted 1-D correlations, n x n per forest
+    const int64_t *__restrict__ wx_off;  // [forest] offset of its matrix in wx
+};
+
+struct WickPairs {
+    const int32_t *__restrict__ a;
+    const int32_t *__restrict__ b;
+    const double *__restrict__ cs;       // cos(ang/2), sin(ang/2)
+    int32_t count;
+};
+
+struct WickWork {
+    int64_t *list;             // per pair, its in-range pixel pairs sorted by bin
+    const int64_t *base;       // [pair] offset of its list (capacity n1 n2)
+    int32_t *count;            // [pair] listed pixel pairs
+    int32_t *task_pair, *task_bin, *task_start, *task_count;
+    unsigned long long *num_tasks;
+    int64_t max_tasks;
+};
+
+struct WickOut {
+    double *weights_wick;                  // [nb]
+    unsigned long long *num_pairs_wick;    // [nb]
+    double *t1, *t2, *t3;                  // [nb][nb]
+    unsigned long long *stats;             // [WICK_NSTATS]
+};
+
+hipError_t launch_wick_list(const WickConst &C, const WickInputs &I, const WickPairs &P,
+                            const WickWork &W, const WickOut &O, hipStream_t stream);
+hipError_t launch_wick_rows(const WickConst &C, const WickInputs &I, const WickPairs &P,
+                            const WickWork &W, const WickOut &O, int64_t tasks,
+                            hipStream_t stream);
+size_t wick_list_lds(const WickConst &C);
+size_t wick_row_lds(const WickConst &C);
+
+// ---------------------------------------------------------------------------------------
 // continuum fit
 // ---------------------------------------------------------------------------------------
 

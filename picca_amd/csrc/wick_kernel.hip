@@ -1,0 +1,250 @@
+// Wick covariance terms T1, T2, T3 of the auto-correlation on gfx950 (MI355X):
+// compute_wickT123_pairs of py/picca/cf.py:1486-1602 for a batch of forest pairs.
+//
+// For each pair the reference lists its in-range pixel pairs (i, j) with bin p, adds
+// w1w2 to weights_wick[p], 1 to num_pairs_wick[p] and w1w2 zev1 zev2 to T1[p, p], and then,
+// for every two listed pairs a < b, adds one product to [p_a, p_b] and [p_b, p_a]:
+//     i_a == i_b:  Wx2[j_a, j_b] w1[i] zev1[i]                        -> T2
+//     j_a == j_b:  Wx1[i_a, i_b] w2[j] zev2[j]                        -> T2
+//     otherwise:   Wx1[i_a, i_b] Wx2[j_a, j_b]                        -> T3
+// (Wx the forests' weighted 1-D correlations).  That is T[P, Q] = sum over a in P, b in Q,
+// b != a, of the product, so:
+//   wick_list_kernel  one workgroup per pair: the in-range pixel pairs sorted by bin
+//                     (LDS counting sort) into a scratch list, the per-bin sums and T1,
+//                     and one task per non-empty bin;
+//   wick_row_kernel   one workgroup per task (pair, bin P): the pixel pairs a of P staged
+//                     in LDS, each thread walks a contiguous run of the pair's list (so b
+//                     is sorted by bin), sums the products over a and adds them to an LDS
+//                     row of T2 / T3 when b's bin changes; the rows go to HBM at the end.
+// FP64 throughout; bins by truncation after divide-then-multiply, as the reference.
+#include <hip/hip_runtime.h>
+
+#include "device_util.h"
+#include "engine.h"
+
+namespace picca {
+
+namespace {
+
+__device__ __forceinline__ void lds_add(double *p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// bin of pixel pair (i, j) or -1 when out of range; r_trans from the comoving distances
+__device__ __forceinline__ int wick_bin(const WickConst &C, double r1, double r2, double c,
+                                        double s) {
+    double r_par = (r1 - r2) * c;
+    if (!C.xcorr) r_par = fabs(r_par);
+    const double r_trans = (r1 + r2) * s;
+    if (!(r_par < C.rpmax && r_trans < C.rtmax && r_par >= C.rpmin)) return -1;
+    const int bp = (int)((r_par - C.rpmin) / C.span * C.np);
+    const int bt = (int)(r_trans / C.rtmax * C.nt);
+    const int p = bt + C.nt * bp;
+    return p < C.nb ? p : -2;   // -2: rounds up to np (the reference writes out of bounds)
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(WICK_BLOCK) void wick_list_kernel(WickConst C, WickInputs I,
+                                                               WickPairs P, WickWork W,
+                                                               WickOut O) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *hw = (double *)smem;          // [nb] sum w1 w2
+    double *ht = hw + C.nb;               // [nb] sum w1 w2 zev1 zev2
+    int *cnt = (int *)(ht + C.nb);        // [nb] pairs per bin, then fill positions
+    __shared__ int s_part[WICK_BLOCK];
+    __shared__ unsigned long long s_over;
+    const int s = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int a = P.a[s], b = P.b[s];
+    const int64_t oa = I.off[a], ob = I.off[b];
+    const int n1 = (int)(I.off[a + 1] - oa), n2 = (int)(I.off[b + 1] - ob);
+    const double c = P.cs[2 * s], sn = P.cs[2 * s + 1];
+    for (int k = tid; k < C.nb; k += WICK_BLOCK) {
+        hw[k] = 0.0;
+        ht[k] = 0.0;
+        cnt[k] = 0;
+    }
+    if (tid == 0) s_over = 0;
+    __syncthreads();
+    const int64_t total = (int64_t)n1 * n2;
+    for (int64_t idx = tid; idx < total; idx += WICK_BLOCK) {
+        const int j = (int)(idx / n1), i = (int)(idx - (int64_t)j * n1);
+        const int p = wick_bin(C, I.r[oa + i], I.r[ob + j], c, sn);
+        if (p == -2) atomicAdd(&s_over, 1ull);
+        if (p < 0) continue;
+        const double w12 = I.w[oa + i] * I.w[ob + j];
+        atomicAdd(&cnt[p], 1);
+        lds_add(hw + p, w12);
+        lds_add(ht + p, w12 * (I.zev1[oa + i] * I.zev2[ob + j]));
+    }
+    __syncthreads();
+    // per-bin sums, T1 and the tasks; then exclusive prefix sum of the counts
+    for (int k = tid; k < C.nb; k += WICK_BLOCK) {
+        if (cnt[k] == 0) continue;
+        unsafeAtomicAdd(O.weights_wick + k, hw[k]);
+        atomicAdd(O.num_pairs_wick + k, (unsigned long long)cnt[k]);
+        unsafeAtomicAdd(O.t1 + (int64_t)k * C.nb + k, ht[k]);
+    }
+    const int per = (C.nb + WICK_BLOCK - 1) / WICK_BLOCK;
+    const int k0 = tid * per, k1 = min(C.nb, k0 + per);
+    int part = 0;
+    for (int k = k0; k < k1; ++k) part += cnt[k];
+    s_part[tid] = part;
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int t = 0; t < WICK_BLOCK; ++t) {
+            const int v = s_part[t];
+            s_part[t] = run;
+            run += v;
+        }
+        W.count[s] = run;
+        if (s_over) atomicAdd(O.stats + WICK_STAT_OVERFLOW, s_over);
+        atomicAdd(O.stats + WICK_STAT_IN_RANGE, (unsigned long long)run);
+    }
+    __syncthreads();
+    int run = s_part[tid];
+    for (int k = k0; k < k1; ++k) {
+        const int v = cnt[k];
+        if (v > 0) {
+            const unsigned long long t = atomicAdd(W.num_tasks, 1ull);
+            if (t < (unsigned long long)W.max_tasks) {
+                W.task_pair[t] = s;
+                W.task_bin[t] = k;
+                W.task_start[t] = run;
+                W.task_count[t] = v;
+            }
+        }
+        cnt[k] = run;   // fill position
+        run += v;
+    }
+    __syncthreads();
+    int64_t *list = W.list + W.base[s];
+    for (int64_t idx = tid; idx < total; idx += WICK_BLOCK) {
+        const int j = (int)(idx / n1), i = (int)(idx - (int64_t)j * n1);
+        const int p = wick_bin(C, I.r[oa + i], I.r[ob + j], c, sn);
+        if (p < 0) continue;
+        const int pos = atomicAdd(&cnt[p], 1);
+        // packed entry: bin (bits 40..63), j (20..39), i (0..19)
+        list[pos] = ((int64_t)p << 40) | ((int64_t)j << 20) | (int64_t)i;
+    }
+}
+
+__global__ __launch_bounds__(WICK_BLOCK) void wick_row_kernel(WickConst C, WickInputs I,
+                                                              WickPairs P, WickWork W,
+                                                              WickOut O) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *r2 = (double *)smem;                  // [nb] row of T2
+    double *r3 = r2 + C.nb;                       // [nb] row of T3
+    int *sa = (int *)(r3 + C.nb);                 // [WICK_STAGE] i, j of staged pairs a
+    int *sj = sa + WICK_STAGE;
+    __shared__ unsigned long long s_products;
+    const int task = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int s = W.task_pair[task];
+    const int pbin = W.task_bin[task];
+    const int a_start = W.task_start[task], a_count = W.task_count[task];
+    const int n_list = W.count[s];
+    const int fa = P.a[s], fb = P.b[s];
+    const int64_t oa = I.off[fa], ob = I.off[fb];
+    const int n1 = (int)(I.off[fa + 1] - oa), n2 = (int)(I.off[fb + 1] - ob);
+    const double *wx1 = I.wx + I.wx_off[fa];
+    const double *wx2 = I.wx + I.wx_off[fb];
+    const int64_t *list = W.list + W.base[s];
+    for (int k = tid; k < C.nb; k += WICK_BLOCK) {
+        r2[k] = 0.0;
+        r3[k] = 0.0;
+    }
+    if (tid == 0) s_products = 0;
+    // this thread's contiguous run of the pair's list (sorted by bin)
+    const int chunk = (n_list + WICK_BLOCK - 1) / WICK_BLOCK;
+    const int b0 = min(n_list, tid * chunk), b1 = min(n_list, b0 + chunk);
+    uint64_t products = 0;
+    for (int s0 = 0; s0 < a_count; s0 += WICK_STAGE) {
+        const int ns = min(WICK_STAGE, a_count - s0);
+        __syncthreads();
+        for (int k = tid; k < ns; k += WICK_BLOCK) {
+            const int64_t e = list[a_start + s0 + k];
+            sa[k] = (int)(e & 0xFFFFF);
+            sj[k] = (int)((e >> 20) & 0xFFFFF);
+        }
+        __syncthreads();
+        int cur = -1;
+        double acc2 = 0.0, acc3 = 0.0;
+        for (int bi = b0; bi < b1; ++bi) {
+            const int64_t e = list[bi];
+            const int ib = (int)(e & 0xFFFFF), jb = (int)((e >> 20) & 0xFFFFF);
+            const int pb = (int)(e >> 40);
+            if (pb != cur) {
+                if (cur >= 0) {
+                    if (acc2 != 0.0) lds_add(r2 + cur, acc2);
+                    if (acc3 != 0.0) lds_add(r3 + cur, acc3);
+                }
+                cur = pb;
+                acc2 = acc3 = 0.0;
+            }
+            const double *row1 = wx1 + (int64_t)ib;   // column ib of Wx1
+            const double *row2 = wx2 + (int64_t)jb;
+            const double w1i = I.w[oa + ib];
+            const double w2j = I.w[ob + jb];
+            const double z1i = I.zev1[oa + ib], z2j = I.zev2[ob + jb];
+            for (int k = 0; k < ns; ++k) {
+                if (pbin == pb && a_start + s0 + k == bi) continue;   // b == a
+                const int ia = sa[k], ja = sj[k];
+                if (ia == ib) {
+                    acc2 += row2[(int64_t)ja * n2] * w1i * z1i;
+                } else if (ja == jb) {
+                    acc2 += row1[(int64_t)ia * n1] * w2j * z2j;
+                } else {
+                    acc3 += row1[(int64_t)ia * n1] * row2[(int64_t)ja * n2];
+                }
+            }
+            products += (uint64_t)ns;
+        }
+        if (cur >= 0) {
+            if (acc2 != 0.0) lds_add(r2 + cur, acc2);
+            if (acc3 != 0.0) lds_add(r3 + cur, acc3);
+        }
+    }
+    atomicAdd(&s_products, (unsigned long long)products);
+    __syncthreads();
+    const int64_t row = (int64_t)pbin * C.nb;
+    for (int k = tid; k < C.nb; k += WICK_BLOCK) {
+        if (r2[k] != 0.0) unsafeAtomicAdd(O.t2 + row + k, r2[k]);
+        if (r3[k] != 0.0) unsafeAtomicAdd(O.t3 + row + k, r3[k]);
+    }
+    if (tid == 0) atomicAdd(O.stats + WICK_STAT_PRODUCTS, s_products);
+}
+
+size_t wick_list_lds(const WickConst &C) { return (size_t)C.nb * (8 + 8 + 4); }
+size_t wick_row_lds(const WickConst &C) {
+    return (size_t)C.nb * 16 + (size_t)WICK_STAGE * 8;
+}
+
+hipError_t launch_wick_list(const WickConst &C, const WickInputs &I, const WickPairs &P,
+                            const WickWork &W, const WickOut &O, hipStream_t stream) {
+    if (P.count <= 0) return hipSuccess;
+    const size_t lds = wick_list_lds(C);
+    hipError_t err = hipFuncSetAttribute((const void *)wick_list_kernel,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(wick_list_kernel, dim3(P.count), dim3(WICK_BLOCK), lds, stream, C, I, P,
+                       W, O);
+    return hipGetLastError();
+}
+
+hipError_t launch_wick_rows(const WickConst &C, const WickInputs &I, const WickPairs &P,
+                            const WickWork &W, const WickOut &O, int64_t tasks,
+                            hipStream_t stream) {
+    if (tasks <= 0) return hipSuccess;
+    const size_t lds = wick_row_lds(C);
+    hipError_t err = hipFuncSetAttribute((const void *)wick_row_kernel,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(wick_row_kernel, dim3((unsigned)tasks), dim3(WICK_BLOCK), lds, stream, C,
+                       I, P, W, O);
+    return hipGetLastError();
+}
+
+}  // namespace picca

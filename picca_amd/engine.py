@@ -253,6 +253,30 @@ class Engine:
                                             ptr(ms)), "picca_cov_subsample")
         return cov, float(ms[0]), float(ms[1])
 
+    # -- Wick covariance terms ---------------------------------------------------
+    def wick_t123(self, params, pair_a, pair_b, pair_ang, zev1, zev2, wx_offsets, wx):
+        """T1, T2, T3 of the resident store's pairs; returns (weights_wick, num_pairs_wick,
+        t1, t2, t3 [nb, nb], report dict)."""
+        nb = params.num_bins_r_par * params.num_bins_r_trans
+        a = as_i32(pair_a)
+        b = as_i32(pair_b)
+        ang = as_f64(pair_ang)
+        zev1 = as_f64(zev1)
+        zev2 = as_f64(zev2)
+        wx_offsets = as_i64(wx_offsets)
+        wx = as_f64(wx)
+        weights_wick = np.zeros(nb)
+        num_pairs_wick = np.zeros(nb, dtype=np.int64)
+        t = [np.zeros(nb * nb) for _ in range(3)]
+        report = _native.WickReport()
+        check(self._lib.picca_wick_t123(self._ctx, ctypes.byref(params), len(a), ptr(a), ptr(b),
+                                        ptr(ang), ptr(zev1), ptr(zev2), ptr(wx_offsets),
+                                        len(wx), ptr(wx), ptr(weights_wick),
+                                        ptr(num_pairs_wick), *(ptr(x) for x in t),
+                                        ctypes.byref(report)), "picca_wick_t123")
+        return (weights_wick, num_pairs_wick, t[0].reshape(nb, nb), t[1].reshape(nb, nb),
+                t[2].reshape(nb, nb), report.as_dict())
+
     # -- metal distortion matrix -------------------------------------------------
     def metal_set_absorber(self, slot, z, r_comov, dist_m, weights, zfac):
         """One absorber's view of every pixel of the resident store (store order)."""

@@ -36,6 +36,13 @@ FILES = {
     "py/picca/tests/data/test_cor/xcf_zcuts.fits.gz": "cor/xcf_zcuts.fits.gz",
     "py/picca/tests/data/test_cor/xdmat.fits.gz": "cor/xdmat.fits.gz",
     "py/picca/tests/data/test_cor/xdmat_zcuts.fits.gz": "cor/xdmat_zcuts.fits.gz",
+    # metal distortion matrices, export, Wick covariance (and its 1-D correlation input)
+    "py/picca/tests/data/test_cor/metal_dmat.fits.gz": "cor/metal_dmat.fits.gz",
+    "py/picca/tests/data/test_cor/metal_dmat_cross.fits.gz": "cor/metal_dmat_cross.fits.gz",
+    "py/picca/tests/data/test_cor/metal_xdmat.fits.gz": "cor/metal_xdmat.fits.gz",
+    "py/picca/tests/data/test_cor/exported_cf.fits.gz": "cor/exported_cf.fits.gz",
+    "py/picca/tests/data/test_cor/wick.fits.gz": "cor/wick.fits.gz",
+    "py/picca/tests/data/test_cor/cf1d.fits.gz": "cor/cf1d.fits.gz",
 }
 
 
