@@ -135,15 +135,20 @@ def main():
                              "counter-derived figure in profiles/ for this workload)")
     parser.add_argument("--mode", default="xi",
                         choices=("xi", "dmat", "continuum", "precision", "xcf", "xdmat",
-                                 "metal", "cov"),
+                                 "metal", "cov", "wick"),
                         help="xi: the pair kernel (default); dmat: the distortion matrix; "
                              "continuum: plain vs learned-seed continuum fits; precision: "
                              "xi error and edge pairs on the heavy-tailed survey; xcf / xdmat: "
                              "forest x quasar correlation / distortion matrix; metal: the "
                              "LYA x SiIII(1207) metal distortion matrix; cov: the "
-                             "subsampling covariance of picca_export")
+                             "subsampling covariance of picca_export; wick: the Wick "
+                             "covariance diagrams 1-3")
     parser.add_argument("--reject", type=float, default=0.99,
                         help="dmat: fraction of neighbour pairs rejected")
+    parser.add_argument("--wick-pairs", type=int, default=64,
+                        help="wick: forest pairs of the workload")
+    parser.add_argument("--wick-cpu-pairs", type=int, default=3,
+                        help="wick: forest pairs of the CPU-baseline sample")
     parser.add_argument("--spectra", type=int, default=200000,
                         help="continuum: spectra fitted (and as many more for training)")
     args = parser.parse_args()
@@ -164,6 +169,8 @@ def main():
         return bench_metal(args)
     if args.mode == "cov":
         return bench_cov(args)
+    if args.mode == "wick":
+        return bench_wick(args)
     return bench_xi(args)
 
 
@@ -325,6 +332,13 @@ class DmatGlobals(Globals):
 class MetalGlobals(Globals):
     num_model_bins_r_par = 50
     num_model_bins_r_trans = 50
+
+
+class WickGlobals(Globals):
+    num_bins_r_par = 15
+    num_bins_r_trans = 15
+    r_par_max = 60.0
+    r_trans_max = 60.0
 
 
 def bench_dmat(args):
@@ -583,6 +597,120 @@ def bench_cov(args):
         result["parity_sample"] = {
             "max_abs_err_over_max": float(np.abs(cov - ref).max() / scale),
             "symmetric": bool(np.array_equal(cov, cov.T))}
+    print(json.dumps(result))
+
+
+def bench_wick(args):
+    """Wick covariance diagrams 1-3 on the 100k-forest survey with the binning of picca's
+    own Wick test (15 x 15 bins to 60 Mpc/h): the first --wick-pairs neighbour pairs of
+    forests kept by one legacy draw each (reject 0.999), a smooth synthetic weighted 1-D
+    correlation per forest (w_k w_l 0.01 exp(-|k - l| / 7), unit variance).  The work per
+    pair grows as the square of its in-range pixel pairs, so the pair count is bounded."""
+    from picca_amd import batch, synth
+    from picca_amd.constants import Cosmo
+    from picca_amd.engine import device_info, get_engine, params_from_globals
+    from picca_amd.utils import compute_ang_max
+
+    survey = synth.make_survey("survey_100k", seed=args.seed, scale=args.scale)
+    store = survey.store
+    cosmo = Cosmo(0.315, verbose=False)
+    ang_max = compute_ang_max(cosmo, WickGlobals.r_trans_max, survey.z_min)
+    pairs = batch.xi_pairs(store, survey.healpix, ang_max)
+    keep_forest = np.random.RandomState(args.seed).rand(store.num_forests) > 0.999
+    keep = np.flatnonzero(keep_forest[pairs.pair_a])[:args.wick_pairs]
+    a, b, ang = pairs.pair_a[keep], pairs.pair_b[keep], pairs.pair_ang[keep]
+    log("wick: %d forests kept, the first %d of their neighbour pairs"
+        % (int(keep_forest.sum()), len(a)))
+    lengths = np.diff(store.offsets)
+    wx_off = np.full(store.num_forests, -1, dtype=np.int64)
+    mats = []
+    used = 0
+    for f in np.unique(np.concatenate([a, b])):
+        n = lengths[f]
+        x = np.arange(n)
+        w = store.weights[store.offsets[f]:store.offsets[f + 1]]
+        mats.append((np.outer(w, w) * (0.01 * np.exp(-np.abs(x[:, None] - x[None, :]) / 7.0))
+                     ).ravel())
+        wx_off[f] = used
+        used += n * n
+    wx = np.concatenate(mats) if mats else np.zeros(0)
+    zev = ((1 + store.z) / (1 + WickGlobals.z_ref)) ** (WickGlobals.alpha - 1)
+    log("wick: weighted 1-D correlations of %d forests (%d values)" % (len(mats), used))
+    params = params_from_globals(WickGlobals)
+    eng = get_engine(0)
+    info = device_info(0)
+    eng.set_forests(store, key="bench_wick")
+    for w in range(args.warmup):
+        eng.wick_t123(params, a, b, ang, zev, zev, wx_off, wx)
+    wall, kernel, rep = [], [], None
+    for k in range(args.steps):
+        t = time.perf_counter()
+        out = eng.wick_t123(params, a, b, ang, zev, zev, wx_off, wx)
+        wall.append(time.perf_counter() - t)
+        rep = out[5]
+        kernel.append(rep["kernel_ms"])
+        log("wick step %d: wall %.1f ms, kernels %.1f ms" % (k, wall[-1] * 1e3, kernel[-1]))
+    kernel_ms = float(np.mean(kernel))
+    products = rep["products"]
+    # per (a, b) product of the reference's loop: the category tests aside, two
+    # multiplications (T2) or one (T3) and the two symmetric additions
+    flops = 3.0 * products
+    achieved = flops / (kernel_ms / 1e3) / 1e12
+    result = {
+        "metric": "Wick pixel-pair products/s (1 GPU)",
+        "value": products / (kernel_ms / 1e3), "unit": "pixel-pair products/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * float(np.mean(wall)),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "wick_100k", "scale": args.scale, "forests": store.num_forests,
+                   "kept_forests": int(keep_forest.sum()), "forest_pairs": int(len(a)),
+                   "reject": 0.999, "bins": "15x15", "r_max": 60.0,
+                   "in_range_pixel_pairs": int(rep["in_range_pairs"]),
+                   "products": int(products), "batches": int(rep["batches"]),
+                   "device": info["name"],
+                   "timing": "value: (a, b) products over the kernels' HIP-event time; "
+                             "ms_per_step includes the upload of the 1-D correlations"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_VALU_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_VALU_PEAK_TFLOPS,
+                     "traffic": None, "flops_per_launch": flops,
+                     "kernel": "picca::wick_row_kernel", "kernel_ms": kernel_ms},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        from oracle import cf_oracle as O
+        # the first pairs of the workload with pixel pairs in range
+        r_par_ok = np.array([
+            np.any(np.abs(store.r_comov[store.offsets[x]:store.offsets[x + 1], None]
+                          - store.r_comov[None, store.offsets[y]:store.offsets[y + 1]])
+                   * np.cos(t / 2) < WickGlobals.r_par_max) for x, y, t in zip(a, b, ang)])
+        sel = np.flatnonzero(r_par_ok)[:args.wick_cpu_pairs]
+        oparams = O.params_from_module(WickGlobals)
+        nb = WickGlobals.num_bins_r_par * WickGlobals.num_bins_r_trans
+        ww = np.zeros(nb)
+        nw = np.zeros(nb, dtype=np.int64)
+        t_ref = [np.zeros(nb * nb) for _ in range(3)]
+        sa, sb, sang = a[sel].astype(np.int32), b[sel].astype(np.int32), ang[sel].copy()
+        tc = time.time()
+        O.lib().oracle_wick_t123(O.ctypes.byref(oparams), len(sel), O._p(sa), O._p(sb),
+                                 O._p(sang), O._p(store.offsets), O._p(store.r_comov),
+                                 O._p(store.weights), O._p(zev), O._p(zev), O._p(wx_off),
+                                 O._p(wx), O._p(ww), O._p(nw), *[O._p(x) for x in t_ref])
+        cpu_s = time.time() - tc
+        mine = eng.wick_t123(params, sa, sb, sang, zev, zev, wx_off, wx)
+        n_ref = float(mine[5]["products"])
+        err = max(float(np.abs(m.ravel() - r).max() / max(np.abs(r).max(), 1e-300))
+                  for m, r in zip(mine[2:5], t_ref))
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        result["cpu_baseline"] = {
+            "value": n_ref / cpu_s, "unit": "pixel-pair products/s", "cores": threads,
+            "kind": "port", "sample": "the first %d forest pairs with pixel pairs in range, "
+                                      "%.3e products, FP64 C "
+                                      "oracle (compute_wickT123_pairs, OpenMP over pairs)"
+                                      % (len(sel), n_ref), "seconds": cpu_s}
+        if n_ref > 0:
+            result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
+        result["parity_sample"] = {"t123_max_abs_err_over_max": err,
+                                   "num_pairs_wick_equal": bool(np.array_equal(mine[1], nw))}
     print(json.dumps(result))
 
 

@@ -12,12 +12,16 @@
 //   wick_list_kernel  one workgroup per pair: the in-range pixel pairs sorted by bin
 //                     (LDS counting sort) into a scratch list, the per-bin sums and T1,
 //                     and one task per non-empty bin;
-//   wick_row_kernel   one workgroup per task (pair, bin P): the pixel pairs a of P staged
-//                     in LDS, each thread walks a contiguous run of the pair's list (so b
-//                     is sorted by bin), sums the products over a and adds them to an LDS
-//                     row of T2 / T3 when b's bin changes; the rows go to HBM at the end.
+//   wick_row_kernel   one workgroup per task (pair, bin P): the pixel pairs a of P on the
+//                     lanes, the pair's list b (sorted by bin) staged through LDS and
+//                     walked in step by the whole wave, so each product reads row ib / jb
+//                     of the symmetric Wx at the lanes' columns ia / ja (neighbouring
+//                     addresses); the lanes' sums are reduced when b's bin changes and added
+//                     to an LDS row of T2 / T3, which goes to HBM at the end.
 // FP64 throughout; bins by truncation after divide-then-multiply, as the reference.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "device_util.h"
 #include "engine.h"
@@ -133,19 +137,23 @@ __global__ __launch_bounds__(WICK_BLOCK) void wick_list_kernel(WickConst C, Wick
 
 __global__ __launch_bounds__(WICK_BLOCK) void wick_row_kernel(WickConst C, WickInputs I,
                                                               WickPairs P, WickWork W,
-                                                              WickOut O) {
+                                                              WickOut O, int split) {
     extern __shared__ __align__(16) unsigned char smem[];
     double *r2 = (double *)smem;                  // [nb] row of T2
     double *r3 = r2 + C.nb;                       // [nb] row of T3
-    int *sa = (int *)(r3 + C.nb);                 // [WICK_STAGE] i, j of staged pairs a
-    int *sj = sa + WICK_STAGE;
+    int64_t *sb = (int64_t *)(r3 + C.nb);         // [WICK_STAGE] staged list entries b
     __shared__ unsigned long long s_products;
-    const int task = blockIdx.x;
+    // workgroup g of `split` for task blockIdx / split takes part g of the pair's list b
+    const int task = blockIdx.x / split;
+    const int g = blockIdx.x - task * split;
     const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
     const int s = W.task_pair[task];
-    const int pbin = W.task_bin[task];
     const int a_start = W.task_start[task], a_count = W.task_count[task];
-    const int n_list = W.count[s];
+    const int n_all = W.count[s];
+    const int part = (n_all + split - 1) / split;
+    const int lo = min(n_all, g * part), hi = min(n_all, lo + part);
     const int fa = P.a[s], fb = P.b[s];
     const int64_t oa = I.off[fa], ob = I.off[fb];
     const int n1 = (int)(I.off[fa + 1] - oa), n2 = (int)(I.off[fb + 1] - ob);
@@ -157,58 +165,69 @@ __global__ __launch_bounds__(WICK_BLOCK) void wick_row_kernel(WickConst C, WickI
         r3[k] = 0.0;
     }
     if (tid == 0) s_products = 0;
-    // this thread's contiguous run of the pair's list (sorted by bin)
-    const int chunk = (n_list + WICK_BLOCK - 1) / WICK_BLOCK;
-    const int b0 = min(n_list, tid * chunk), b1 = min(n_list, b0 + chunk);
+    // work units: (chunk of 64 pixel pairs a of bin P on the lanes, part of the list b)
+    const int achunks = (a_count + 63) >> 6;
+    const int bsplit = max(1, (WICK_BLOCK / 64) / achunks);
+    const int units = achunks * bsplit;
     uint64_t products = 0;
-    for (int s0 = 0; s0 < a_count; s0 += WICK_STAGE) {
-        const int ns = min(WICK_STAGE, a_count - s0);
+    for (int s0 = lo; s0 < hi; s0 += WICK_STAGE) {
+        const int ns = min(WICK_STAGE, hi - s0);
         __syncthreads();
-        for (int k = tid; k < ns; k += WICK_BLOCK) {
-            const int64_t e = list[a_start + s0 + k];
-            sa[k] = (int)(e & 0xFFFFF);
-            sj[k] = (int)((e >> 20) & 0xFFFFF);
-        }
+        for (int k = tid; k < ns; k += WICK_BLOCK) sb[k] = list[s0 + k];
         __syncthreads();
-        int cur = -1;
-        double acc2 = 0.0, acc3 = 0.0;
-        for (int bi = b0; bi < b1; ++bi) {
-            const int64_t e = list[bi];
-            const int ib = (int)(e & 0xFFFFF), jb = (int)((e >> 20) & 0xFFFFF);
-            const int pb = (int)(e >> 40);
-            if (pb != cur) {
-                if (cur >= 0) {
-                    if (acc2 != 0.0) lds_add(r2 + cur, acc2);
-                    if (acc3 != 0.0) lds_add(r3 + cur, acc3);
+        for (int u = wave; u < units; u += WICK_BLOCK / 64) {
+            const int c = u / bsplit, q = u - c * bsplit;
+            const int ai = c * 64 + lane;
+            const bool live = ai < a_count;
+            int ia = 0, ja = 0;
+            double w1i = 0.0, w2j = 0.0, z1i = 0.0, z2j = 0.0;
+            if (live) {
+                const int64_t e = list[a_start + ai];
+                ia = (int)(e & 0xFFFFF);
+                ja = (int)((e >> 20) & 0xFFFFF);
+                w1i = I.w[oa + ia];
+                z1i = I.zev1[oa + ia];
+                w2j = I.w[ob + ja];
+                z2j = I.zev2[ob + ja];
+            }
+            const int per = (ns + bsplit - 1) / bsplit;
+            const int k0 = min(ns, q * per), k1 = min(ns, k0 + per);
+            int cur = -1;
+            double acc2 = 0.0, acc3 = 0.0;
+            auto flush = [&]() {
+                const double t2v = wave_sum_f64(acc2), t3v = wave_sum_f64(acc3);
+                if (lane == 0) {
+                    if (t2v != 0.0) lds_add(r2 + cur, t2v);
+                    if (t3v != 0.0) lds_add(r3 + cur, t3v);
                 }
-                cur = pb;
                 acc2 = acc3 = 0.0;
-            }
-            const double *row1 = wx1 + (int64_t)ib;   // column ib of Wx1
-            const double *row2 = wx2 + (int64_t)jb;
-            const double w1i = I.w[oa + ib];
-            const double w2j = I.w[ob + jb];
-            const double z1i = I.zev1[oa + ib], z2j = I.zev2[ob + jb];
-            for (int k = 0; k < ns; ++k) {
-                if (pbin == pb && a_start + s0 + k == bi) continue;   // b == a
-                const int ia = sa[k], ja = sj[k];
+            };
+            for (int k = k0; k < k1; ++k) {
+                const int64_t e = sb[k];
+                const int pb = (int)(e >> 40);
+                if (pb != cur) {
+                    if (cur >= 0) flush();
+                    cur = pb;
+                }
+                if (!live || a_start + ai == s0 + k) continue;   // b == a
+                const int ib = (int)(e & 0xFFFFF), jb = (int)((e >> 20) & 0xFFFFF);
+                // Wx is symmetric: row ib (the same for every lane), column ia
                 if (ia == ib) {
-                    acc2 += row2[(int64_t)ja * n2] * w1i * z1i;
+                    acc2 += wx2[(int64_t)jb * n2 + ja] * w1i * z1i;
                 } else if (ja == jb) {
-                    acc2 += row1[(int64_t)ia * n1] * w2j * z2j;
+                    acc2 += wx1[(int64_t)ib * n1 + ia] * w2j * z2j;
                 } else {
-                    acc3 += row1[(int64_t)ia * n1] * row2[(int64_t)ja * n2];
+                    acc3 += wx1[(int64_t)ib * n1 + ia] * wx2[(int64_t)jb * n2 + ja];
                 }
             }
-            products += (uint64_t)ns;
-        }
-        if (cur >= 0) {
-            if (acc2 != 0.0) lds_add(r2 + cur, acc2);
-            if (acc3 != 0.0) lds_add(r3 + cur, acc3);
+            if (cur >= 0) flush();
+            if (live) products += (uint64_t)(k1 - k0);
         }
     }
-    atomicAdd(&s_products, (unsigned long long)products);
+    const uint64_t wp = wave_sum_u64(products);
+    if (lane == 0) atomicAdd(&s_products, (unsigned long long)wp);
     __syncthreads();
+    const int pbin = W.task_bin[task];
     const int64_t row = (int64_t)pbin * C.nb;
     for (int k = tid; k < C.nb; k += WICK_BLOCK) {
         if (r2[k] != 0.0) unsafeAtomicAdd(O.t2 + row + k, r2[k]);
@@ -242,8 +261,11 @@ hipError_t launch_wick_rows(const WickConst &C, const WickInputs &I, const WickP
     hipError_t err = hipFuncSetAttribute((const void *)wick_row_kernel,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(wick_row_kernel, dim3((unsigned)tasks), dim3(WICK_BLOCK), lds, stream, C,
-                       I, P, W, O);
+    // enough workgroups for about eight waves per SIMD: each task's list split in parts
+    const int64_t want = 8 * 1024 / (WICK_BLOCK / 64);
+    const int split = (int)std::max<int64_t>(1, std::min<int64_t>(64, (want + tasks - 1) / tasks));
+    hipLaunchKernelGGL(wick_row_kernel, dim3((unsigned)(tasks * split)), dim3(WICK_BLOCK), lds,
+                       stream, C, I, P, W, O, split);
     return hipGetLastError();
 }
 
