@@ -248,6 +248,11 @@ struct MetalConst {
     int32_t shp;        // remove same-half-plate close pairs
     double rpmin, rpmax, rtmax, span, width_p;
     double inv_den;     // 1 / the evolution factor's denominator
+    // bins per Mpc/h (data r_par, r_trans, model r_par, r_trans) for the reciprocal-multiply
+    // bin coordinates, and 0.5 - the margin around an integer inside which a coordinate is
+    // recomputed with the reference's divide-then-multiply
+    double inv_p, inv_t, inv_pm, inv_tm;
+    double half_p, half_t, half_pm, half_tm;
 };
 
 struct MetalPairs {

@@ -110,6 +110,16 @@ int picca_metal_pass(picca_ctx *ctx, const picca_cf_params *params, int32_t mode
     C.span = params->r_par_max - params->r_par_min;
     C.width_p = C.span / np;
     C.inv_den = 1.0 / zw_den;
+    C.inv_p = np / C.span;
+    C.inv_t = nt / C.rtmax;
+    C.inv_pm = npm / C.span;
+    C.inv_tm = ntm / C.rtmax;
+    // x * (n / den) and x / den * n differ by at most 4 ulp of |t| < n + 1 for any
+    // coordinate that can land in range; the margin n 2^-46 + 1e-12 covers that 16 times over
+    C.half_p = 0.5 - (np * 0x1p-46 + 1e-12);
+    C.half_t = 0.5 - (nt * 0x1p-46 + 1e-12);
+    C.half_pm = 0.5 - (npm * 0x1p-46 + 1e-12);
+    C.half_tm = 0.5 - (ntm * 0x1p-46 + 1e-12);
     const int64_t nb = C.nb, nm = C.nm;
 
     PICCA_HIP_CHECK(hipSetDevice(ctx->device));

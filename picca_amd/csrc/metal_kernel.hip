@@ -12,11 +12,12 @@
 // the second group.  It is a histogram with no continuum-projection terms, so the kernel is
 // a pair loop like the correlation's: one wave per (pair, 64 pixels of delta1), lanes over
 // the pixels i, a scalar walk over the pixels j of delta2 inside the window that can give an
-// in-range data bin.  Consecutive pairs of a lane with the same (A, M) merge into one run;
-// a run adds to weights_dmat and the four effective sums in LDS (privatised per workgroup
-// when they fit) and to dmat in HBM.  All arithmetic is FP64 with the reference's
-// divide-then-multiply bins; pixel weights are 0 where the absorber's redshift is not below
-// the quasar's (the reference drops those pixels).
+// in-range data bin.  Pairs with the same (A, M) merge into runs that travel along the
+// diagonal (i + 1, j + 1) from lane to lane (a DPP rotation of the wave); a run adds to weights_dmat and the four
+// effective sums in LDS (privatised per workgroup when they fit) and to dmat in HBM.  All
+// arithmetic is FP64; the bins are the reference's divide-then-multiply ones (a reciprocal
+// multiply away from bin edges); pixel weights are 0 where the absorber's redshift is not
+// below the quasar's (the reference drops those pixels).
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -61,15 +62,30 @@ __device__ __forceinline__ void metal_flush(const MetalRun &run, const MetalCons
 }
 
 // Flat bin of (r_par, r_trans) with n_par x n_trans bins, or -1 when out of range
-// (py/picca/cf.py:1036-1052 / py/picca/xcf.py:733-742).  METAL_XCF counts a bin that
-// rounds up to n_par (the reference's bincount would fail there) in `over`.
+// (py/picca/cf.py:1036-1052 / py/picca/xcf.py:733-742): floor of the r_par coordinate,
+// truncation of the r_trans coordinate (numpy's astype(int)).  The coordinates come from a
+// multiply by bins per Mpc/h; when one lies within the margin (0.5 - half) of an integer,
+// both are recomputed with the reference's divide-then-multiply, so the bin is the
+// reference's.  METAL_XCF counts a bin that rounds up to n_par (the reference's bincount
+// would fail there) in `over`.
 template <int MODE>
 __device__ __forceinline__ int metal_bin(double r_par, double r_trans, const MetalConst &C,
-                                         int n_par, int n_trans, uint64_t &over) {
+                                         int n_par, int n_trans, double inv_p, double inv_t,
+                                         double half_p, double half_t, uint32_t &over) {
+    if (MODE == METAL_XCF && !(r_par > C.rpmin && r_par < C.rpmax && r_trans < C.rtmax))
+        return -1;
+    const double x = r_par - C.rpmin;
+    double tp = x * inv_p, tt = r_trans * inv_t;
+    double fp = floor(tp), ft = floor(tt);
+    if ((fabs(tp - fp - 0.5) > half_p) | (fabs(tt - ft - 0.5) > half_t)) {
+        tp = x / C.span * n_par;
+        tt = r_trans / C.rtmax * n_trans;
+        fp = floor(tp);
+    }
+    ft = trunc(tt);
     if (MODE == METAL_XCF) {
-        if (!(r_par > C.rpmin && r_par < C.rpmax && r_trans < C.rtmax)) return -1;
-        const int bp = (int)((r_par - C.rpmin) / C.span * n_par);
-        const int bt = (int)(r_trans / C.rtmax * n_trans);
+        const int bp = (int)fp;
+        const int bt = (int)ft;
         const int bin = bt + n_trans * bp;
         if (bin >= n_par * n_trans) {
             ++over;
@@ -77,11 +93,38 @@ __device__ __forceinline__ int metal_bin(double r_par, double r_trans, const Met
         }
         return bt >= 0 ? bin : -1;   // a negative r_trans (never for physical inputs)
     } else {
-        const double bp = floor((r_par - C.rpmin) / C.span * n_par);
-        const double bt = trunc(r_trans / C.rtmax * n_trans);
-        if (!(bp < n_par && bt < n_trans && bp >= 0 && bt >= 0)) return -1;
-        return (int)bt + n_trans * (int)bp;
+        if (!(fp < n_par && ft < n_trans && fp >= 0 && ft >= 0)) return -1;
+        return (int)ft + n_trans * (int)fp;
     }
+}
+
+// Reads a wave-uniform value through the constant address space (a scalar load).
+__device__ __forceinline__ double scalar_load(const double *p) {
+    return *(const __attribute__((address_space(4))) double *)p;
+}
+
+// Lane l receives lane l - 1's value, lane 0 lane 63's (DPP wave_ror:1 over the full wave).
+// Called with every lane of the wave active.
+__device__ __forceinline__ int rotate_lanes(int v) {
+    return __builtin_amdgcn_mov_dpp(v, 0x13c, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ double rotate_lanes(double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)rotate_lanes((int)(uint32_t)b);
+    const uint32_t hi = (uint32_t)rotate_lanes((int)(uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+__device__ __forceinline__ MetalRun rotate_run(const MetalRun &r) {
+    return MetalRun{rotate_lanes(r.a), rotate_lanes(r.m), rotate_lanes(r.w),
+                    rotate_lanes(r.wz), rotate_lanes(r.rp), rotate_lanes(r.rt),
+                    rotate_lanes(r.z)};
+}
+
+// x == 0 (either sign) from the bits, so that a uniform value is tested on the scalar unit
+__device__ __forceinline__ bool is_zero(double x) {
+    return ((uint64_t)__double_as_longlong(x) << 1) == 0;
 }
 
 }  // namespace
@@ -101,6 +144,7 @@ __global__ __launch_bounds__(METAL_BLOCK) void metal_kernel(
         __syncthreads();
     }
     uint64_t n_data = 0, n_model = 0, n_over = 0, n_visit = 0;
+    uint32_t over = 0;
     for (;;) {
         int64_t t = 0;
         if (lane == 0) t = (int64_t)atomicAdd(O.queue, 1ull);
@@ -153,43 +197,73 @@ __global__ __launch_bounds__(METAL_BLOCK) void metal_kernel(
             }
         }
 
+        // Accumulators travel along the diagonal: after each pixel j every lane hands its
+        // run to the next lane (lane 63 to lane 0), so lane l + 1 meets at pixel j + 1 the
+        // run that lane l left at pixel j.  On a common wavelength grid (i + 1, j + 1) is
+        // almost the separation of (i, j), so a run stays in one (A, M) cell for many
+        // pixels and the matrix sees one atomic per run instead of one per lane and cell.
+        // A lane flushes the run it receives when its own in-range pair falls in another
+        // cell, and passes it on unchanged when it has no in-range pair; every run is
+        // flushed at the end of the task.
         MetalRun run{-1, -1, 0, 0, 0, 0, 0};
+        uint32_t visit = 0, data = 0, model = 0;   // this task's counts (j < 2^31)
         for (int j = j_lo; j < j_hi; ++j) {
             const int64_t q = ob + j;
-            const double w2 = TB.w[q];
-            if (w2 == 0) continue;
-            const double r2 = Fr[q], d2 = Fd[q];
-            const double z2a = TB.z[q], r2a = TB.r[q], d2a = TB.d[q];
-            const double g2 = TB.f[q] * C.inv_den;
-            if (!live) continue;
-            ++n_visit;
-            double r_par = (r1 - r2) * cth;
-            if (MODE == METAL_AUTO) r_par = fabs(r_par);
-            const double r_trans = (d1 + d2) * sth;
-            const int A = metal_bin<MODE>(r_par, r_trans, C, C.np, C.nt, n_over);
-            if (A < 0) continue;
-            ++n_data;
-            double w12 = w1 * w2;
-            if (shp && fabs(r_par) < C.width_p) w12 = 0.0;
-            double r_par_a = (r1a - r2a) * cth;
-            if (MODE == METAL_AUTO) r_par_a = fabs(r_par_a);
-            const double r_trans_a = (d1a + d2a) * sth;
-            const int M = metal_bin<MODE>(r_par_a, r_trans_a, C, C.npm, C.ntm, n_over);
-            if (A != run.a || M != run.m) {
-                metal_flush<LDS_HIST>(run, C, h_wdm, h_eff, O);
-                run = MetalRun{A, M, 0, 0, 0, 0, 0};
+            const double w2 = scalar_load(TB.w + q);
+            if (is_zero(w2)) continue;
+            const double r2 = scalar_load(Fr + q), d2 = scalar_load(Fd + q);
+            const double z2a = scalar_load(TB.z + q), r2a = scalar_load(TB.r + q);
+            const double d2a = scalar_load(TB.d + q);
+            const double g2 = scalar_load(TB.f + q) * C.inv_den;
+            int A = -1, M = -1;
+            double c_w = 0, c_wz = 0, c_rp = 0, c_rt = 0, c_z = 0;
+            if (live) {
+                ++visit;
+                double r_par = (r1 - r2) * cth;
+                if (MODE == METAL_AUTO) r_par = fabs(r_par);
+                const double r_trans = (d1 + d2) * sth;
+                A = metal_bin<MODE>(r_par, r_trans, C, C.np, C.nt, C.inv_p, C.inv_t, C.half_p,
+                                    C.half_t, over);
+                if (A >= 0) {
+                    ++data;
+                    double w12 = w1 * w2;
+                    if (shp && fabs(r_par) < C.width_p) w12 = 0.0;
+                    c_w = w12;
+                    double r_par_a = (r1a - r2a) * cth;
+                    if (MODE == METAL_AUTO) r_par_a = fabs(r_par_a);
+                    const double r_trans_a = (d1a + d2a) * sth;
+                    M = metal_bin<MODE>(r_par_a, r_trans_a, C, C.npm, C.ntm, C.inv_pm,
+                                        C.inv_tm, C.half_pm, C.half_tm, over);
+                    if (M >= 0) {
+                        ++model;
+                        const double wzw = w12 * (f1a * g2);
+                        c_wz = wzw;
+                        c_rp = r_par_a * wzw;
+                        c_rt = r_trans_a * wzw;
+                        c_z = (z1a + z2a) / 2 * wzw;
+                    }
+                }
             }
-            run.w += w12;
-            if (M < 0) continue;
-            ++n_model;
-            const double zw = f1a * g2;
-            run.wz += w12 * zw;
-            run.rp += r_par_a * w12 * zw;
-            run.rt += r_trans_a * w12 * zw;
-            run.z += (z1a + z2a) / 2 * w12 * zw;
+            if (A >= 0) {
+                if (A != run.a || M != run.m) {
+                    metal_flush<LDS_HIST>(run, C, h_wdm, h_eff, O);
+                    run = MetalRun{A, M, c_w, c_wz, c_rp, c_rt, c_z};
+                } else {
+                    run.w += c_w;
+                    run.wz += c_wz;
+                    run.rp += c_rp;
+                    run.rt += c_rt;
+                    run.z += c_z;
+                }
+            }
+            run = rotate_run(run);
         }
         metal_flush<LDS_HIST>(run, C, h_wdm, h_eff, O);
+        n_visit += visit;
+        n_data += data;
+        n_model += model;
     }
+    n_over = over;
 
     const uint64_t v_data = wave_sum_u64(n_data);
     const uint64_t v_model = wave_sum_u64(n_model);
