@@ -152,6 +152,7 @@ struct picca_ctx {
         bool set = false;
         int64_t pixels = 0;
         DeviceBuffer<double> z, r, d, w, f;
+        DeviceBuffer<picca::MetalRecord> rec;   // num_pixels + METAL_REC_PAD records
     };
     MetalSlot metal[4];
     DeviceBuffer<int64_t> m_task_begin;

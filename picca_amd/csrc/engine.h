@@ -239,7 +239,24 @@ struct MetalTable {
     const double *__restrict__ d;
     const double *__restrict__ w;
     const double *__restrict__ f;
+    const struct MetalRecord *__restrict__ rec;   // the same pixels as one record each
 };
+
+// One pixel of an absorber table with its forest's Lya coordinates, read as one 64-byte
+// scalar load by the pixel loop (the table holds METAL_REC_PAD zero records past the end,
+// so that the loop can load the next record unconditionally).
+struct MetalRecord {
+    double r, d;    // Lya r_comov, dist_m
+    double w;       // pixel weight (0 where the absorber's redshift is not below the quasar's)
+    double za, ra, da;   // absorber redshift, r_comov, dist_m
+    double f;       // redshift-evolution factor
+    double pad;
+};
+static_assert(sizeof(MetalRecord) == 64, "MetalRecord layout");
+constexpr int METAL_REC_PAD = 4;
+
+hipError_t launch_metal_pack(const double *r, const double *d, const MetalTable &T,
+                             int64_t num_pixels, MetalRecord *rec, hipStream_t stream);
 
 struct MetalConst {
     int32_t np, nt, nb;

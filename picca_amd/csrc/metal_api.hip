@@ -31,6 +31,10 @@ int picca_metal_set_absorber(picca_ctx *ctx, int32_t slot, const double *z,
     PICCA_HIP_CHECK(m.d.upload(dist_m, (size_t)npix, ctx->stream));
     PICCA_HIP_CHECK(m.w.upload(weights, (size_t)npix, ctx->stream));
     PICCA_HIP_CHECK(m.f.upload(zfac, (size_t)npix, ctx->stream));
+    PICCA_HIP_CHECK(m.rec.reserve((size_t)npix + picca::METAL_REC_PAD));
+    const picca::MetalTable T{m.z.ptr, m.r.ptr, m.d.ptr, m.w.ptr, m.f.ptr, nullptr};
+    PICCA_HIP_CHECK(picca::launch_metal_pack(ctx->r.ptr, ctx->d.ptr, T, npix, m.rec.ptr,
+                                             ctx->stream));
     PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     m.set = true;
     m.pixels = npix;
@@ -144,8 +148,8 @@ int picca_metal_pass(picca_ctx *ctx, const picca_cf_params *params, int32_t mode
 
     const picca_ctx::MetalSlot &sa = ctx->metal[slot_a];
     const picca_ctx::MetalSlot &sb = ctx->metal[slot_b];
-    picca::MetalTable TA{sa.z.ptr, sa.r.ptr, sa.d.ptr, sa.w.ptr, sa.f.ptr};
-    picca::MetalTable TB{sb.z.ptr, sb.r.ptr, sb.d.ptr, sb.w.ptr, sb.f.ptr};
+    picca::MetalTable TA{sa.z.ptr, sa.r.ptr, sa.d.ptr, sa.w.ptr, sa.f.ptr, sa.rec.ptr};
+    picca::MetalTable TB{sb.z.ptr, sb.r.ptr, sb.d.ptr, sb.w.ptr, sb.f.ptr, sb.rec.ptr};
     picca::MetalPairs P{ctx->dpa.ptr, ctx->dpb.ptr, ctx->dcs.ptr, ctx->dflags.ptr,
                         ctx->m_task_begin.ptr, num_pairs, tasks};
     picca::MetalOut O{ctx->o_wdm.ptr, ctx->o_dmat.ptr, ctx->o_eff.ptr, ctx->o_eff.ptr + nm,

@@ -122,6 +122,22 @@ __device__ __forceinline__ MetalRun rotate_run(const MetalRun &r) {
                     rotate_lanes(r.z)};
 }
 
+// Record of a wave-uniform pixel through the constant address space (scalar loads).
+__device__ __forceinline__ MetalRecord load_metal_record(const MetalRecord *p) {
+    typedef const __attribute__((address_space(4))) MetalRecord ConstRecord;
+    const ConstRecord *q = (const ConstRecord *)p;
+    MetalRecord out;
+    out.r = q->r;
+    out.d = q->d;
+    out.w = q->w;
+    out.za = q->za;
+    out.ra = q->ra;
+    out.da = q->da;
+    out.f = q->f;
+    out.pad = 0;
+    return out;
+}
+
 // x == 0 (either sign) from the bits, so that a uniform value is tested on the scalar unit
 __device__ __forceinline__ bool is_zero(double x) {
     return ((uint64_t)__double_as_longlong(x) << 1) == 0;
@@ -207,56 +223,61 @@ __global__ __launch_bounds__(METAL_BLOCK) void metal_kernel(
         // flushed at the end of the task.
         MetalRun run{-1, -1, 0, 0, 0, 0, 0};
         uint32_t visit = 0, data = 0, model = 0;   // this task's counts (j < 2^31)
+        const double f1s = f1a * C.inv_den;
+        // pixel j's record is in SGPRs; the next one loads while this one is evaluated (the
+        // table's padding keeps the load of record j_hi in bounds)
+        const MetalRecord *rec = TB.rec + ob;
+        MetalRecord cur = load_metal_record(rec + j_lo);
+        // wait for the first record here, so that the loop only ever waits for a record it
+        // requested an iteration earlier
+        asm volatile("" ::"s"(cur.r), "s"(cur.w), "s"(cur.ra), "s"(cur.f));
         for (int j = j_lo; j < j_hi; ++j) {
-            const int64_t q = ob + j;
-            const double w2 = scalar_load(TB.w + q);
-            if (is_zero(w2)) continue;
-            const double r2 = scalar_load(Fr + q), d2 = scalar_load(Fd + q);
-            const double z2a = scalar_load(TB.z + q), r2a = scalar_load(TB.r + q);
-            const double d2a = scalar_load(TB.d + q);
-            const double g2 = scalar_load(TB.f + q) * C.inv_den;
-            int A = -1, M = -1;
-            double c_w = 0, c_wz = 0, c_rp = 0, c_rt = 0, c_z = 0;
-            if (live) {
-                ++visit;
-                double r_par = (r1 - r2) * cth;
-                if (MODE == METAL_AUTO) r_par = fabs(r_par);
-                const double r_trans = (d1 + d2) * sth;
-                A = metal_bin<MODE>(r_par, r_trans, C, C.np, C.nt, C.inv_p, C.inv_t, C.half_p,
-                                    C.half_t, over);
-                if (A >= 0) {
-                    ++data;
-                    double w12 = w1 * w2;
-                    if (shp && fabs(r_par) < C.width_p) w12 = 0.0;
-                    c_w = w12;
-                    double r_par_a = (r1a - r2a) * cth;
-                    if (MODE == METAL_AUTO) r_par_a = fabs(r_par_a);
-                    const double r_trans_a = (d1a + d2a) * sth;
-                    M = metal_bin<MODE>(r_par_a, r_trans_a, C, C.npm, C.ntm, C.inv_pm,
-                                        C.inv_tm, C.half_pm, C.half_tm, over);
-                    if (M >= 0) {
-                        ++model;
-                        const double wzw = w12 * (f1a * g2);
-                        c_wz = wzw;
-                        c_rp = r_par_a * wzw;
-                        c_rt = r_trans_a * wzw;
-                        c_z = (z1a + z2a) / 2 * wzw;
+            const MetalRecord nxt = load_metal_record(rec + j + 1);
+            if (!is_zero(cur.w)) {
+                int A = -1, M = -1;
+                double c_w = 0, c_wz = 0, c_rp = 0, c_rt = 0, c_z = 0;
+                if (live) {
+                    ++visit;
+                    double r_par = (r1 - cur.r) * cth;
+                    if (MODE == METAL_AUTO) r_par = fabs(r_par);
+                    const double r_trans = (d1 + cur.d) * sth;
+                    A = metal_bin<MODE>(r_par, r_trans, C, C.np, C.nt, C.inv_p, C.inv_t,
+                                        C.half_p, C.half_t, over);
+                    if (A >= 0) {
+                        ++data;
+                        double w12 = w1 * cur.w;
+                        if (shp && fabs(r_par) < C.width_p) w12 = 0.0;
+                        c_w = w12;
+                        double r_par_a = (r1a - cur.ra) * cth;
+                        if (MODE == METAL_AUTO) r_par_a = fabs(r_par_a);
+                        const double r_trans_a = (d1a + cur.da) * sth;
+                        M = metal_bin<MODE>(r_par_a, r_trans_a, C, C.npm, C.ntm, C.inv_pm,
+                                            C.inv_tm, C.half_pm, C.half_tm, over);
+                        if (M >= 0) {
+                            ++model;
+                            const double wzw = w12 * (f1s * cur.f);
+                            c_wz = wzw;
+                            c_rp = r_par_a * wzw;
+                            c_rt = r_trans_a * wzw;
+                            c_z = (z1a + cur.za) / 2 * wzw;
+                        }
                     }
                 }
-            }
-            if (A >= 0) {
-                if (A != run.a || M != run.m) {
-                    metal_flush<LDS_HIST>(run, C, h_wdm, h_eff, O);
-                    run = MetalRun{A, M, c_w, c_wz, c_rp, c_rt, c_z};
-                } else {
-                    run.w += c_w;
-                    run.wz += c_wz;
-                    run.rp += c_rp;
-                    run.rt += c_rt;
-                    run.z += c_z;
+                if (A >= 0) {
+                    if (A != run.a || M != run.m) {
+                        metal_flush<LDS_HIST>(run, C, h_wdm, h_eff, O);
+                        run = MetalRun{A, M, c_w, c_wz, c_rp, c_rt, c_z};
+                    } else {
+                        run.w += c_w;
+                        run.wz += c_wz;
+                        run.rp += c_rp;
+                        run.rt += c_rt;
+                        run.z += c_z;
+                    }
                 }
+                run = rotate_run(run);
             }
-            run = rotate_run(run);
+            cur = nxt;
         }
         metal_flush<LDS_HIST>(run, C, h_wdm, h_eff, O);
         n_visit += visit;
@@ -285,6 +306,33 @@ __global__ __launch_bounds__(METAL_BLOCK) void metal_kernel(
             if (v != 0) unsafeAtomicAdd(eff[k / C.nm] + (k % C.nm), v);
         }
     }
+}
+
+__global__ __launch_bounds__(256) void metal_pack_kernel(const double *__restrict__ r,
+                                                         const double *__restrict__ d,
+                                                         MetalTable T, int64_t num_pixels,
+                                                         MetalRecord *__restrict__ rec) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= num_pixels + METAL_REC_PAD) return;
+    MetalRecord out{0, 0, 0, 0, 0, 0, 0, 0};
+    if (p < num_pixels) {
+        out.r = r[p];
+        out.d = d[p];
+        out.w = T.w[p];
+        out.za = T.z[p];
+        out.ra = T.r[p];
+        out.da = T.d[p];
+        out.f = T.f[p];
+    }
+    rec[p] = out;
+}
+
+hipError_t launch_metal_pack(const double *r, const double *d, const MetalTable &T,
+                             int64_t num_pixels, MetalRecord *rec, hipStream_t stream) {
+    const int64_t n = num_pixels + METAL_REC_PAD;
+    hipLaunchKernelGGL(metal_pack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       stream, r, d, T, num_pixels, rec);
+    return hipGetLastError();
 }
 
 size_t metal_lds_bytes(const MetalConst &C) {
