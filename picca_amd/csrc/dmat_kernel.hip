@@ -180,6 +180,24 @@ __device__ void window_a(const DmatConst &C, const DmatPair &q, const DmatForest
            q.sin_half > 0, lane, lo, hi);
 }
 
+// Narrows [lo, hi) to the other forest's pixels whose pairs with a pixel at dist_m d_self
+// can have r_trans in data row t or in model row u: r_trans = (d_self + d_other) sin(ang/2)
+// grows with the other forest's (monotone) dist_m, so the two rows bound d_other.  The
+// slack is far above the rounding of the row edges and of r_trans.
+__device__ void row_window(const DmatConst &C, const DmatPair &q, const double *__restrict__ Od,
+                           int n, double d_self, int t, int u, int lane, int &lo, int &hi) {
+    if (!q.prune || !(q.sin_half > 0) || lo >= hi) return;
+    const double r_lo = fmin(t * C.rtmax / C.nt_d, u * C.rtmax / C.ntm_d);
+    const double r_hi = fmax((t + 1) * C.rtmax / C.nt_d, (u + 1) * C.rtmax / C.ntm_d);
+    const double d_lo = r_lo / q.sin_half - d_self;
+    const double d_hi = r_hi / q.sin_half - d_self;
+    const double slack = 1e-6 + 1e-9 * (fabs(r_hi / q.sin_half) + fabs(d_self));
+    const int l = uniform_int(wave_search<true>(Od, n, d_lo - slack, lane));
+    const int h = uniform_int(wave_search<false>(Od, n, d_hi + slack, lane));
+    lo = max(lo, l);
+    hi = min(hi, h);
+}
+
 __device__ __forceinline__ void lds_add(double *p, double v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
@@ -743,6 +761,7 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
                                 const bool i_sel = !(q.zerr_on && zerr_cut(C, z1, q.zq2));
                                 int j_lo, j_hi;
                                 window_b(C, q, F, r1, r1, d1, lane, j_lo, j_hi);
+                                row_window(C, q, Bd, q.lb, d1, t, u, lane, j_lo, j_hi);
                                 for (int j = j_lo + lane; j < j_hi; j += 64) {
                                     const double w2 = Bw[j];
                                     if (w2 == 0) continue;
@@ -854,6 +873,7 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
                                 const bool j_zerr = q.zerr_on && zerr_cut(C, z2, q.zq1);
                                 int i_lo, i_hi;
                                 window_a(C, q, F, r2, d2, lane, i_lo, i_hi);
+                                row_window(C, q, Ad, q.la, d2, t, u, lane, i_lo, i_hi);
                                 for (int i = i_lo + lane; i < i_hi; i += 64) {
                                     const double w1 = Aw[i];
                                     if (w1 == 0) continue;
