@@ -48,6 +48,9 @@ struct DmatPair {
     int64_t oa, ob;
     double ang, cos_half, sin_half;
     double zq1, zq2, sum_w1, sum_w2, sw1, sw2;
+    // reciprocals of the continuum-fit normalisations: the eta terms multiply by these
+    // instead of dividing per pixel pair (a few ulp from the reference's divisions)
+    double inv_w1, inv_w2, inv_s1, inv_s2;
     bool shp_cut;   // remove_same_half_plate_close_pairs and the pair is on one half plate
     bool zerr_on;   // zerr cut active for this pair (ang < zerr_cut_deg)
     bool order1, order2;
@@ -74,6 +77,10 @@ __device__ DmatPair load_pair(const DmatConst &C, const DmatForests &F, const Dm
     q.sum_w2 = uniform_f64(F.sum_w[q.b]);
     q.sw1 = uniform_f64(F.sw_dll[q.a]);
     q.sw2 = uniform_f64(F.sw_dll[q.b]);
+    q.inv_w1 = 1.0 / q.sum_w1;
+    q.inv_w2 = 1.0 / q.sum_w2;
+    q.inv_s1 = 1.0 / q.sw1;
+    q.inv_s2 = 1.0 / q.sw2;
     q.shp_cut = (C.flags & XI_REMOVE_SHP) &&
                 (uniform_int(P.flags[s]) & PAIR_SAME_HALF_PLATE) != 0;
     q.zerr_on = (C.flags & XI_HAS_ZERR) && (q.ang < C.zerr_ang);
@@ -521,7 +528,7 @@ __global__ __launch_bounds__(DMAT_PASS_BLOCK) void dmat_pass_kernel(DmatConst C,
             }
             const bool i_sel = !(q.zerr_on && zerr_cut(C, z1, q.zq2));
             // per-pixel factors of the eta sums (py/picca/cf.py:765-843)
-            const double a7 = q.order1 ? w1 * l1 / q.sw1 : 0.0;    // w1 dll1 / sum w dll^2
+            const double a7 = q.order1 ? w1 * l1 * q.inv_s1 : 0.0;    // w1 dll1 / sum w dll^2
             int j_lo, j_hi;
             window_b(C, q, F, wave_min(active ? r1 : INFINITY),
                      wave_max(active ? r1 : -INFINITY), wave_min(active ? d1 : INFINITY), lane,
@@ -548,11 +555,11 @@ __global__ __launch_bounds__(DMAT_PASS_BLOCK) void dmat_pass_kernel(DmatConst C,
                     if (rm.key >= 0) flush_m(rm);
                     rm = RunM{M, 0, 0, 0, 0, 0, 0, 0, 0};
                 }
-                if (q.proj2) rm.e5 += zfac * w12 / q.sum_w1 / q.sum_w2;
-                if (q.order2) rm.e6 += zfac * w1 / q.sum_w1 * (w2 * l2 / q.sw2);
+                if (q.proj2) rm.e5 += zfac * w12 * q.inv_w1 * q.inv_w2;
+                if (q.order2) rm.e6 += zfac * w1 * q.inv_w1 * (w2 * l2 * q.inv_s2);
                 if (q.order1) {
-                    if (q.proj2) rm.e7 += zfac * w2 / q.sum_w2 * a7;
-                    if (q.order2) rm.e8 += zfac * w1 * l1 * w2 * l2 / q.sw1 / q.sw2;
+                    if (q.proj2) rm.e7 += zfac * w2 * q.inv_w2 * a7;
+                    if (q.order2) rm.e8 += zfac * w1 * l1 * w2 * l2 * q.inv_s1 * q.inv_s2;
                 }
                 if (!sel) continue;
                 rm.rp += w12 * r_par;
@@ -777,9 +784,9 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
                                     const int m = M / C.ntm - mrp0;
                                     const bool in_m = mu == u && m >= 0 && m < nmrp;
                                     if (in_m && mask[m] != 0 && q.proj2) {
-                                        lds_add(vec + VEC_E + m, zfac * w2 / q.sum_w2);
+                                        lds_add(vec + VEC_E + m, zfac * w2 * q.inv_w2);
                                         if (q.order2)
-                                            lds_add(vec + VEC_E2 + m, zfac * w2 * l2 / q.sw2);
+                                            lds_add(vec + VEC_E2 + m, zfac * w2 * l2 * q.inv_s2);
                                     }
                                     const bool j_zerr = q.zerr_on && zerr_cut(C, z2, q.zq1);
                                     if (!pair_selected(C, q, i_sel, j_zerr, (z1 + z2) / 2, r_par))
@@ -836,8 +843,8 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
                             const int m = M / C.ntm - mrp0;
                             const bool in_m = mu == u && m >= 0 && m < nmrp;
                             if (in_m && mask[m] != 0) {
-                                lds_add(sv + VEC_E + m, zfac * w1 / q.sum_w1);
-                                if (q.order1) lds_add(sv + VEC_E2 + m, zfac * w1 * l1 / q.sw1);
+                                lds_add(sv + VEC_E + m, zfac * w1 * q.inv_w1);
+                                if (q.order1) lds_add(sv + VEC_E2 + m, zfac * w1 * l1 * q.inv_s1);
                             }
                             const bool i_sel = !(q.zerr_on && zerr_cut(C, z1, q.zq2));
                             if (!pair_selected(C, q, i_sel, j_zerr, (z1 + z2) / 2, r_par))
@@ -888,9 +895,9 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
                                     const int mu = M % C.ntm;
                                     const int m = M / C.ntm - mrp0;
                                     if (mu == u && m >= 0 && m < nmrp && mask[m] != 0) {
-                                        lds_add(vec + VEC_E + m, zfac * w1 / q.sum_w1);
+                                        lds_add(vec + VEC_E + m, zfac * w1 * q.inv_w1);
                                         if (q.order1)
-                                            lds_add(vec + VEC_E2 + m, zfac * w1 * l1 / q.sw1);
+                                            lds_add(vec + VEC_E2 + m, zfac * w1 * l1 * q.inv_s1);
                                     }
                                     const bool i_sel = !(q.zerr_on && zerr_cut(C, z1, q.zq2));
                                     if (!pair_selected(C, q, i_sel, j_zerr, (z1 + z2) / 2, r_par))
