@@ -163,9 +163,10 @@ struct picca_ctx {
     DeviceBuffer<double> cv_xi, cv_w, cv_y, cv_s, cv_cov;
 
     // Wick terms: evolution factors, weighted 1-D correlations, per-batch lists and tasks
-    DeviceBuffer<double> wk_zev1, wk_zev2, wk_wx;
-    DeviceBuffer<int64_t> wk_wxoff, wk_base, wk_list;
-    DeviceBuffer<int32_t> wk_count, wk_task;
+    DeviceBuffer<double> wk_zev1, wk_zev2, wk_wx, wk_c1;
+    DeviceBuffer<int64_t> wk_wxoff, wk_base, wk_list, wk_c1off;
+    DeviceBuffer<int32_t> wk_count, wk_task, wk_segcount, wk_forests;
+    DeviceBuffer<int4> wk_seg;
     DeviceBuffer<unsigned long long> wk_nb, wk_stats, wk_ntasks;
 
     // continuum fit: resident spectra (forest pixels) and per-forest results

@@ -336,6 +336,8 @@ struct WickInputs {
     const double *__restrict__ zev2;     // ((1+z)/(1+z_ref))^(alpha2-1)
     const double *__restrict__ wx;       // weighted 1-D correlations, n x n per forest
     const int64_t *__restrict__ wx_off;  // [forest] offset of its matrix in wx
+    const double *__restrict__ c1;       // row prefix sums of wx, n x (n + 1) per forest
+    const int64_t *__restrict__ c1_off;  // [forest] offset of its prefix sums in c1
 };
 
 struct WickPairs {
@@ -349,6 +351,10 @@ struct WickWork {
     int64_t *list;             // per pair, its in-range pixel pairs sorted by bin
     const int64_t *base;       // [pair] offset of its list (capacity n1 n2)
     int32_t *count;            // [pair] listed pixel pairs
+    int4 *seg;                 // per pair (same offsets as list), its segments sorted by bin:
+                               // (bin, j, i_lo, i_hi), pixels i_lo..i_hi of delta1 that
+                               // pixel j of delta2 pairs with in that bin
+    int32_t *seg_count;        // [pair] segments
     int32_t *task_pair, *task_bin, *task_start, *task_count;
     unsigned long long *num_tasks;
     int64_t max_tasks;
@@ -368,6 +374,10 @@ hipError_t launch_wick_rows(const WickConst &C, const WickInputs &I, const WickP
                             hipStream_t stream);
 size_t wick_list_lds(const WickConst &C);
 size_t wick_row_lds(const WickConst &C);
+// row prefix sums c1 of the weighted 1-D correlation of each listed forest
+hipError_t launch_wick_prefix(const double *wx, const int64_t *wx_off, const int64_t *off,
+                              const int32_t *forests, int32_t num_forests, double *c1,
+                              const int64_t *c1_off, hipStream_t stream);
 
 // ---------------------------------------------------------------------------------------
 // continuum fit

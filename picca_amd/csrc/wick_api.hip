@@ -80,12 +80,35 @@ int picca_wick_t123(picca_ctx *ctx, const picca_cf_params *params, int64_t num_p
     PICCA_HIP_CHECK(picca::launch_zero_u64((unsigned long long *)ctx->o_dmat.ptr, 3 * nb * nb,
                                            ctx->stream));
     PICCA_HIP_CHECK(picca::launch_zero_u64(ctx->wk_stats.ptr, picca::WICK_NSTATS, ctx->stream));
-    picca::WickInputs I{ctx->off.ptr, ctx->r.ptr, ctx->w.ptr, ctx->wk_zev1.ptr,
-                        ctx->wk_zev2.ptr, ctx->wk_wx.ptr, ctx->wk_wxoff.ptr};
+    // row prefix sums of the weighted 1-D correlation of every forest of the pairs
+    std::vector<int64_t> c1off((size_t)nf, 0);
+    std::vector<int32_t> forests;
+    {
+        std::vector<char> used((size_t)nf, 0);
+        for (int64_t k = 0; k < num_pairs; ++k) used[(size_t)pair_a[k]] = used[(size_t)pair_b[k]] = 1;
+        int64_t total = 0;
+        for (int64_t f = 0; f < nf; ++f) {
+            if (!used[(size_t)f]) continue;
+            const int64_t n = off[f + 1] - off[f];
+            c1off[(size_t)f] = total;
+            total += n * (n + 1);
+            forests.push_back((int32_t)f);
+        }
+        PICCA_HIP_CHECK(ctx->wk_c1.reserve((size_t)std::max<int64_t>(total, 1)));
+    }
+    PICCA_HIP_CHECK(ctx->wk_c1off.upload(c1off.data(), c1off.size(), ctx->stream));
+    PICCA_HIP_CHECK(ctx->wk_forests.upload(forests.data(), forests.size(), ctx->stream));
+    PICCA_HIP_CHECK(picca::launch_wick_prefix(ctx->wk_wx.ptr, ctx->wk_wxoff.ptr, ctx->off.ptr,
+                                              ctx->wk_forests.ptr, (int32_t)forests.size(),
+                                              ctx->wk_c1.ptr, ctx->wk_c1off.ptr, ctx->stream));
+    picca::WickInputs I{ctx->off.ptr,      ctx->r.ptr,        ctx->w.ptr,
+                        ctx->wk_zev1.ptr,  ctx->wk_zev2.ptr,  ctx->wk_wx.ptr,
+                        ctx->wk_wxoff.ptr, ctx->wk_c1.ptr,    ctx->wk_c1off.ptr};
     picca::WickOut O{ctx->o_wdm.ptr, ctx->wk_nb.ptr, ctx->o_dmat.ptr, ctx->o_dmat.ptr + nb * nb,
                      ctx->o_dmat.ptr + 2 * nb * nb, ctx->wk_stats.ptr};
 
-    // batches of pairs whose lists (n1 n2 entries each, an upper bound) fit 1 GiB
+    // batches of pairs whose lists (n1 n2 entries each, an upper bound) fit 1 GiB (and their
+    // segment lists, 16 bytes per entry at most, 2 GiB)
     const int64_t list_budget = (int64_t)1 << 27;
     double total_ms = 0.0;
     int64_t batches = 0;
@@ -114,14 +137,18 @@ int picca_wick_t123(picca_ctx *ctx, const picca_cf_params *params, int64_t num_p
         PICCA_HIP_CHECK(ctx->dcs.upload(cs.data(), cs.size(), ctx->stream));
         PICCA_HIP_CHECK(ctx->wk_base.upload(base.data(), base.size(), ctx->stream));
         PICCA_HIP_CHECK(ctx->wk_list.reserve((size_t)std::max<int64_t>(entries, 1)));
+        PICCA_HIP_CHECK(ctx->wk_seg.reserve((size_t)std::max<int64_t>(entries, 1)));
         PICCA_HIP_CHECK(ctx->wk_count.reserve((size_t)count));
+        PICCA_HIP_CHECK(ctx->wk_segcount.reserve((size_t)count));
         PICCA_HIP_CHECK(ctx->wk_task.reserve((size_t)(4 * std::max<int64_t>(max_tasks, 1))));
         PICCA_HIP_CHECK(picca::launch_zero_u64(ctx->wk_ntasks.ptr, 1, ctx->stream));
         int32_t *tk = ctx->wk_task.ptr;
         const int64_t mt = std::max<int64_t>(max_tasks, 1);
         picca::WickPairs P{ctx->dpa.ptr, ctx->dpb.ptr, ctx->dcs.ptr, (int32_t)count};
-        picca::WickWork W{ctx->wk_list.ptr, ctx->wk_base.ptr, ctx->wk_count.ptr, tk, tk + mt,
-                          tk + 2 * mt, tk + 3 * mt, ctx->wk_ntasks.ptr, max_tasks};
+        picca::WickWork W{ctx->wk_list.ptr,  ctx->wk_base.ptr,     ctx->wk_count.ptr,
+                          ctx->wk_seg.ptr,   ctx->wk_segcount.ptr, tk,
+                          tk + mt,           tk + 2 * mt,          tk + 3 * mt,
+                          ctx->wk_ntasks.ptr, max_tasks};
         PICCA_HIP_CHECK(hipEventRecord(ctx->ev_start, ctx->stream));
         PICCA_HIP_CHECK(picca::launch_wick_list(C, I, P, W, O, ctx->stream));
         unsigned long long tasks = 0;
