@@ -348,7 +348,9 @@ typedef struct picca_wick_report {
  * weights; zev1 / zev2 [store pixels] are ((1+z)/(1+z_ref))^(alpha-1) and ^(alpha2-1);
  * wx + wx_offsets[f] is forest f's n x n weighted 1-D correlation
  * w_k w_l xi_1d(|log_lambda_k - log_lambda_l|) sqrt(v_k v_l) (py/picca/cf.py:1392-1398),
- * needed for every forest of the pair list.  Outputs (accumulated): weights_wick[nb],
+ * symmetric (its rows are read as its columns), needed for every forest of the pair list.
+ * Device memory per call: the row prefix sums of those matrices (n (n+1) doubles each) and
+ * per batch up to 24 bytes per (pixel, pixel) of its pairs.  Outputs (accumulated): weights_wick[nb],
  * num_pairs_wick[nb], t1, t2, t3 [nb][nb].  report may be NULL.
  */
 int picca_wick_t123(picca_ctx *ctx, const picca_cf_params *params, int64_t num_pairs,
