@@ -350,8 +350,8 @@ typedef struct picca_wick_report {
  * w_k w_l xi_1d(|log_lambda_k - log_lambda_l|) sqrt(v_k v_l) (py/picca/cf.py:1392-1398),
  * symmetric (its rows are read as its columns), needed for every forest of the pair list.
  * Device memory per call: the row prefix sums of those matrices (n (n+1) doubles each) and
- * per batch up to 24 bytes per (pixel, pixel) of its pairs.  Outputs (accumulated): weights_wick[nb],
- * num_pairs_wick[nb], t1, t2, t3 [nb][nb].  report may be NULL.
+ * per batch up to 24 bytes per (pixel, pixel) of its pairs.  Outputs (accumulated):
+ * weights_wick[nb], num_pairs_wick[nb], t1, t2, t3 [nb][nb].  report may be NULL.
  */
 int picca_wick_t123(picca_ctx *ctx, const picca_cf_params *params, int64_t num_pairs,
                     const int32_t *pair_a, const int32_t *pair_b, const double *pair_ang,
