@@ -12,16 +12,17 @@
 //   wick_list_kernel  one workgroup per pair: the in-range pixel pairs sorted by bin
 //                     (LDS counting sort) into a scratch list, the per-bin sums and T1,
 //                     and one task per non-empty bin;
-//   wick_row_kernel   one workgroup per task (pair, bin P): the pixel pairs a of P on the
-//                     lanes; the partners b come as segments (bin Q, j_b, i_lo..i_hi):
+//   wick_row_kernel   one workgroup per task (pair, bin P) and part of the pair's partner
+//                     segments b (bin Q, j_b, i_lo..i_hi), one segment per lane:
 //                     the pixels of delta1 that pixel j_b pairs with in Q, consecutive.
 //                     Over a segment the products share Wx2[j_a, j_b] (or w2 zev2), so
 //                     their sum needs only sum_{i_b = i_lo..i_hi} Wx1[i_a, i_b], one
 //                     difference of the row prefix sums c1 of Wx1 (minus the i_b == i_a
-//                     term, which is a T2 product or a itself).  The segments (sorted by
-//                     bin) are staged through LDS and walked in step by the whole wave;
-//                     the lanes' sums are reduced when Q changes and added to an LDS row
-//                     of T2 / T3, which goes to HBM at the end.
+//                     term, which is a T2 product or a itself).  The pixel pairs a of P
+//                     are staged through LDS and walked in step by the wave, so each step
+//                     reads one row of c1 (i_a) and one row of the symmetric Wx2 (j_a);
+//                     each lane's sums go to an LDS row of T2 / T3 (at its segment's bin),
+//                     which goes to HBM at the end.
 // FP64 throughout; bins by truncation after divide-then-multiply, as the reference.
 #include <hip/hip_runtime.h>
 
@@ -165,13 +166,22 @@ __global__ __launch_bounds__(WICK_BLOCK) void wick_list_kernel(WickConst C, Wick
     }
 }
 
+// pixel pair a of the task's bin as the row kernel stages it
+struct WickA {
+    int32_t ia, ja;
+    double w1z1, w2z2;   // w1 zev1 at i_a, w2 zev2 at j_a
+    double d1;           // Wx1[i_a, i_a]
+};
+static_assert(sizeof(WickA) == 32, "WickA layout");
+constexpr int WICK_ASTAGE = 512;   // pixel pairs a staged in LDS at a time
+
 __global__ __launch_bounds__(WICK_BLOCK) void wick_row_kernel(WickConst C, WickInputs I,
                                                               WickPairs P, WickWork W,
                                                               WickOut O, int split) {
     extern __shared__ __align__(16) unsigned char smem[];
     double *r2 = (double *)smem;                  // [nb] row of T2
     double *r3 = r2 + C.nb;                       // [nb] row of T3
-    int4 *sb = (int4 *)(r3 + C.nb);               // [WICK_STAGE] staged segments
+    WickA *sa = (WickA *)(r3 + C.nb);             // [WICK_ASTAGE] staged pixel pairs a
     __shared__ unsigned long long s_products;
     // workgroup g of `split` for task blockIdx / split takes part g of the pair's segments
     const int task = blockIdx.x / split;
@@ -197,65 +207,52 @@ __global__ __launch_bounds__(WICK_BLOCK) void wick_row_kernel(WickConst C, WickI
         r3[k] = 0.0;
     }
     if (tid == 0) s_products = 0;
-    // work units: (chunk of 64 pixel pairs a of bin P on the lanes, part of the segments)
-    const int achunks = (a_count + 63) >> 6;
-    const int bsplit = max(1, (WICK_BLOCK / 64) / achunks);
-    const int units = achunks * bsplit;
+    // Lanes own segments b; the wave walks the staged pixel pairs a in step, so every read
+    // of a step is in one row: row i_a of the prefix sums and row j_a of the symmetric Wx2.
     uint64_t products = 0;
-    for (int s0 = lo; s0 < hi; s0 += WICK_STAGE) {
-        const int ns = min(WICK_STAGE, hi - s0);
+    for (int a0 = 0; a0 < a_count; a0 += WICK_ASTAGE) {
+        const int na = min(WICK_ASTAGE, a_count - a0);
         __syncthreads();
-        for (int k = tid; k < ns; k += WICK_BLOCK) sb[k] = segs[s0 + k];
+        for (int k = tid; k < na; k += WICK_BLOCK) {
+            const int64_t e = list[a_start + a0 + k];
+            WickA A;
+            A.ia = (int)(e & 0xFFFFF);
+            A.ja = (int)((e >> 20) & 0xFFFFF);
+            A.w1z1 = I.w[oa + A.ia] * I.zev1[oa + A.ia];
+            A.w2z2 = I.w[ob + A.ja] * I.zev2[ob + A.ja];
+            A.d1 = wx1[(int64_t)A.ia * n1 + A.ia];
+            sa[k] = A;
+        }
         __syncthreads();
-        for (int u = wave; u < units; u += WICK_BLOCK / 64) {
-            const int c = u / bsplit, q = u - c * bsplit;
-            const int ai = c * 64 + lane;
-            const bool live = ai < a_count;
-            int ia = 0, ja = 0;
-            double w1z1 = 0.0, w2z2 = 0.0, d1 = 0.0;
-            const double *c1row = c1;
-            if (live) {
-                const int64_t e = list[a_start + ai];
-                ia = (int)(e & 0xFFFFF);
-                ja = (int)((e >> 20) & 0xFFFFF);
-                w1z1 = I.w[oa + ia] * I.zev1[oa + ia];
-                w2z2 = I.w[ob + ja] * I.zev2[ob + ja];
-                d1 = wx1[(int64_t)ia * n1 + ia];
-                c1row = c1 + (int64_t)ia * (n1 + 1);
-            }
-            const int per = (ns + bsplit - 1) / bsplit;
-            const int k0 = min(ns, q * per), k1 = min(ns, k0 + per);
-            int cur = -1;
+        for (int c0 = lo + wave * 64; c0 < hi; c0 += WICK_BLOCK) {
+            const int sidx = c0 + lane;
+            const bool live = sidx < hi;
+            // (bin, j_b, i_lo, i_hi); a dead lane's empty segment reads c1[i_a][0] twice
+            const int4 e = live ? segs[sidx] : make_int4(0, -1, 0, -1);
             double acc2 = 0.0, acc3 = 0.0;
-            auto flush = [&]() {
-                const double t2v = wave_sum_f64(acc2), t3v = wave_sum_f64(acc3);
-                if (lane == 0) {
-                    if (t2v != 0.0) lds_add(r2 + cur, t2v);
-                    if (t3v != 0.0) lds_add(r3 + cur, t3v);
-                }
-                acc2 = acc3 = 0.0;
-            };
-            for (int k = k0; k < k1; ++k) {
-                const int4 e = sb[k];   // (bin, j_b, i_lo, i_hi)
-                if (e.x != cur) {
-                    if (cur >= 0) flush();
-                    cur = e.x;
-                }
-                if (!live) continue;
+            uint32_t cnt = 0;
+            for (int k = 0; k < na; ++k) {
+                const WickA A = sa[k];
+                const double *crow = c1 + (int64_t)A.ia * (n1 + 1);
                 // sum of Wx1[i_a, i_b] over the segment without i_b == i_a
-                const bool in = ia >= e.z && ia <= e.w;
-                double s1 = c1row[e.w + 1] - c1row[e.z];
-                if (in) s1 -= d1;
-                products += (uint64_t)(e.w - e.z + 1) - ((in && e.y == ja) ? 1 : 0);
-                if (e.y == ja) {
-                    acc2 += s1 * w2z2;   // same j: Wx1[i_a, i_b] w2 zev2
-                } else {
-                    const double x2 = wx2[(int64_t)e.y * n2 + ja];
-                    acc3 += x2 * s1;                // i_b != i_a: Wx1 Wx2
-                    if (in) acc2 += x2 * w1z1;      // i_b == i_a: Wx2[j_a, j_b] w1 zev1
+                const bool in = A.ia >= e.z && A.ia <= e.w;
+                double s1 = crow[e.w + 1] - crow[e.z];
+                if (in) s1 -= A.d1;
+                const bool same_j = e.y == A.ja;
+                cnt += (uint32_t)(e.w - e.z + 1) - ((in && same_j) ? 1u : 0u);
+                if (same_j) {
+                    acc2 += s1 * A.w2z2;   // same j: Wx1[i_a, i_b] w2 zev2
+                } else if (live) {
+                    const double x2 = wx2[(int64_t)A.ja * n2 + e.y];
+                    acc3 += x2 * s1;                 // i_b != i_a: Wx1 Wx2
+                    if (in) acc2 += x2 * A.w1z1;     // i_b == i_a: Wx2[j_a, j_b] w1 zev1
                 }
             }
-            if (cur >= 0) flush();
+            products += cnt;
+            if (live) {
+                if (acc2 != 0.0) lds_add(r2 + e.x, acc2);
+                if (acc3 != 0.0) lds_add(r3 + e.x, acc3);
+            }
         }
     }
     const uint64_t wp = wave_sum_u64(products);
@@ -304,7 +301,7 @@ hipError_t launch_wick_prefix(const double *wx, const int64_t *wx_off, const int
 
 size_t wick_list_lds(const WickConst &C) { return (size_t)C.nb * (8 + 8 + 4 + 4); }
 size_t wick_row_lds(const WickConst &C) {
-    return (size_t)C.nb * 16 + (size_t)WICK_STAGE * sizeof(int4);
+    return (size_t)C.nb * 16 + (size_t)WICK_ASTAGE * sizeof(WickA);
 }
 
 hipError_t launch_wick_list(const WickConst &C, const WickInputs &I, const WickPairs &P,
