@@ -231,22 +231,22 @@ __global__ __launch_bounds__(WICK_BLOCK) void wick_row_kernel(WickConst C, WickI
             const int4 e = live ? segs[sidx] : make_int4(0, -1, 0, -1);
             double acc2 = 0.0, acc3 = 0.0;
             uint32_t cnt = 0;
+#pragma unroll 4
             for (int k = 0; k < na; ++k) {
                 const WickA A = sa[k];
                 const double *crow = c1 + (int64_t)A.ia * (n1 + 1);
                 // sum of Wx1[i_a, i_b] over the segment without i_b == i_a
                 const bool in = A.ia >= e.z && A.ia <= e.w;
+                const bool same_j = e.y == A.ja;
+                // Wx2[j_a, j_b], 0 for j_b == j_a (those products are T2) and dead lanes
+                double x2 = 0.0;
+                if (live && !same_j) x2 = wx2[(int64_t)A.ja * n2 + e.y];
                 double s1 = crow[e.w + 1] - crow[e.z];
                 if (in) s1 -= A.d1;
-                const bool same_j = e.y == A.ja;
                 cnt += (uint32_t)(e.w - e.z + 1) - ((in && same_j) ? 1u : 0u);
-                if (same_j) {
-                    acc2 += s1 * A.w2z2;   // same j: Wx1[i_a, i_b] w2 zev2
-                } else if (live) {
-                    const double x2 = wx2[(int64_t)A.ja * n2 + e.y];
-                    acc3 += x2 * s1;                 // i_b != i_a: Wx1 Wx2
-                    if (in) acc2 += x2 * A.w1z1;     // i_b == i_a: Wx2[j_a, j_b] w1 zev1
-                }
+                acc3 += x2 * s1;   // i_b != i_a, j_b != j_a: Wx1 Wx2
+                // same j: Wx1[i_a, i_b] w2 zev2; i_b == i_a: Wx2[j_a, j_b] w1 zev1
+                acc2 += same_j ? s1 * A.w2z2 : (in ? x2 * A.w1z1 : 0.0);
             }
             products += cnt;
             if (live) {
