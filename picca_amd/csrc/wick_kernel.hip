@@ -231,7 +231,7 @@ __global__ __launch_bounds__(WICK_BLOCK) void wick_row_kernel(WickConst C, WickI
             const int4 e = live ? segs[sidx] : make_int4(0, -1, 0, -1);
             double acc2 = 0.0, acc3 = 0.0;
             uint32_t cnt = 0;
-#pragma unroll 4
+#pragma unroll 8
             for (int k = 0; k < na; ++k) {
                 const WickA A = sa[k];
                 const double *crow = c1 + (int64_t)A.ia * (n1 + 1);
