@@ -148,8 +148,8 @@ __global__ __launch_bounds__(WICK_BLOCK) void wick_list_kernel(WickConst C, Wick
     __syncthreads();
     int64_t *list = W.list + W.base[s];
     int4 *seg = W.seg + W.base[s];
-    // i-major, so that a bin's list keeps the pairs of one pixel i together: the lanes of
-    // the row kernel then share few rows of the prefix sums
+    // i-major, so that a bin's list keeps the pairs of one pixel i together: consecutive
+    // steps of the row kernel then read the same row of the prefix sums
     for (int64_t idx = tid; idx < total; idx += WICK_BLOCK) {
         const int i = (int)(idx / n2), j = (int)(idx - (int64_t)i * n2);
         const double rj = I.r[ob + j];
