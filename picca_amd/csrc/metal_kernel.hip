@@ -234,47 +234,42 @@ __global__ __launch_bounds__(METAL_BLOCK) void metal_kernel(
         for (int j = j_lo; j < j_hi; ++j) {
             const MetalRecord nxt = load_metal_record(rec + j + 1);
             if (!is_zero(cur.w)) {
-                int A = -1, M = -1;
-                double c_w = 0, c_wz = 0, c_rp = 0, c_rt = 0, c_z = 0;
-                if (live) {
-                    ++visit;
-                    double r_par = (r1 - cur.r) * cth;
-                    if (MODE == METAL_AUTO) r_par = fabs(r_par);
-                    const double r_trans = (d1 + cur.d) * sth;
-                    A = metal_bin<MODE>(r_par, r_trans, C, C.np, C.nt, C.inv_p, C.inv_t,
-                                        C.half_p, C.half_t, over);
-                    if (A >= 0) {
-                        ++data;
-                        double w12 = w1 * cur.w;
-                        if (shp && fabs(r_par) < C.width_p) w12 = 0.0;
-                        c_w = w12;
-                        double r_par_a = (r1a - cur.ra) * cth;
-                        if (MODE == METAL_AUTO) r_par_a = fabs(r_par_a);
-                        const double r_trans_a = (d1a + cur.da) * sth;
-                        M = metal_bin<MODE>(r_par_a, r_trans_a, C, C.npm, C.ntm, C.inv_pm,
-                                            C.inv_tm, C.half_pm, C.half_tm, over);
-                        if (M >= 0) {
-                            ++model;
-                            const double wzw = w12 * (f1s * cur.f);
-                            c_wz = wzw;
-                            c_rp = r_par_a * wzw;
-                            c_rt = r_trans_a * wzw;
-                            c_z = (z1a + cur.za) / 2 * wzw;
-                        }
-                    }
+                // branch-free pair: every lane evaluates both bins, the conditions select
+                double r_par = (r1 - cur.r) * cth;
+                if (MODE == METAL_AUTO) r_par = fabs(r_par);
+                const double r_trans = (d1 + cur.d) * sth;
+                uint32_t over_a = 0;
+                int A = metal_bin<MODE>(r_par, r_trans, C, C.np, C.nt, C.inv_p, C.inv_t,
+                                        C.half_p, C.half_t, over_a);
+                double r_par_a = (r1a - cur.ra) * cth;
+                if (MODE == METAL_AUTO) r_par_a = fabs(r_par_a);
+                const double r_trans_a = (d1a + cur.da) * sth;
+                uint32_t over_m = 0;
+                int M = metal_bin<MODE>(r_par_a, r_trans_a, C, C.npm, C.ntm, C.inv_pm,
+                                        C.inv_tm, C.half_pm, C.half_tm, over_m);
+                A = live ? A : -1;
+                const bool in_a = A >= 0;
+                if (!in_a) M = -1;
+                const bool in_m = M >= 0;
+                // a data-bin overflow counts for a live lane, a model-bin one for a data pair
+                over += (live ? over_a : 0u) + (in_a ? over_m : 0u);
+                visit += live ? 1u : 0u;
+                data += in_a ? 1u : 0u;
+                model += in_m ? 1u : 0u;
+                double w12 = w1 * cur.w;
+                if (shp && fabs(r_par) < C.width_p) w12 = 0.0;
+                const double c_w = in_a ? w12 : 0.0;
+                const double wzw = in_m ? w12 * (f1s * cur.f) : 0.0;
+                if (in_a && (A != run.a || M != run.m)) {
+                    metal_flush<LDS_HIST>(run, C, h_wdm, h_eff, O);
+                    run = MetalRun{A, M, 0, 0, 0, 0, 0};
                 }
-                if (A >= 0) {
-                    if (A != run.a || M != run.m) {
-                        metal_flush<LDS_HIST>(run, C, h_wdm, h_eff, O);
-                        run = MetalRun{A, M, c_w, c_wz, c_rp, c_rt, c_z};
-                    } else {
-                        run.w += c_w;
-                        run.wz += c_wz;
-                        run.rp += c_rp;
-                        run.rt += c_rt;
-                        run.z += c_z;
-                    }
-                }
+                // a pair out of range adds zeros to the run it passes on
+                run.w += c_w;
+                run.wz += wzw;
+                run.rp += r_par_a * wzw;
+                run.rt += r_trans_a * wzw;
+                run.z += (z1a + cur.za) / 2 * wzw;
                 run = rotate_run(run);
             }
             cur = nxt;
