@@ -448,6 +448,10 @@ def bench_metal(args):
     eng.set_forests(store, key="bench_metal")
     eng.metal_set_absorber(0, *lya)
     eng.metal_set_absorber(1, *si)
+    if args.traffic_bytes < 0:
+        args.traffic_bytes = measured_traffic("metal_dmat_100k", args.scale, "metal_kernel")
+    if args.copy_probe > 0:
+        copy_probe(eng, args.copy_probe)
     nb = MetalGlobals.num_bins_r_par * MetalGlobals.num_bins_r_trans
     nm = MetalGlobals.num_model_bins_r_par * MetalGlobals.num_model_bins_r_trans
 
@@ -491,7 +495,8 @@ def bench_metal(args):
                              "and the 50 MB matrix download of each pass"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_VALU_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_VALU_PEAK_TFLOPS,
-                     "traffic": None, "flops_per_launch": flops / 2, "launches_per_step": 2,
+                     "traffic": (args.traffic_bytes / 2 if args.traffic_bytes > 0 else None),
+                     "flops_per_launch": flops / 2, "launches_per_step": 2,
                      "launch_avg_ms": kernel_ms / 2, "kernel": "picca::metal_kernel",
                      "kernel_ms": kernel_ms},
         "cpu_baseline": None,
