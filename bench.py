@@ -645,6 +645,10 @@ def bench_wick(args):
     eng = get_engine(0)
     info = device_info(0)
     eng.set_forests(store, key="bench_wick")
+    if args.traffic_bytes < 0:
+        args.traffic_bytes = measured_traffic("wick_100k", args.scale, "wick_kernels")
+    if args.copy_probe > 0:
+        copy_probe(eng, args.copy_probe)
     for w in range(args.warmup):
         eng.wick_t123(params, a, b, ang, zev, zev, wx_off, wx)
     wall, kernel, rep = [], [], None
@@ -677,7 +681,8 @@ def bench_wick(args):
                              "ms_per_step includes the upload of the 1-D correlations"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_VALU_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_VALU_PEAK_TFLOPS,
-                     "traffic": None, "flops_per_launch": flops,
+                     "traffic": args.traffic_bytes if args.traffic_bytes > 0 else None,
+                     "flops_per_launch": flops,
                      "kernel": "picca::wick_row_kernel", "kernel_ms": kernel_ms},
         "cpu_baseline": None,
     }
