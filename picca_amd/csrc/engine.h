@@ -313,7 +313,6 @@ hipError_t launch_cov_syrk(const double *xi, const double *w, const double *S,
 // ---------------------------------------------------------------------------------------
 
 constexpr int WICK_BLOCK = 256;
-constexpr int WICK_STAGE = 1024;   // pixel pairs a staged in LDS at a time
 
 enum : int {
     WICK_STAT_IN_RANGE = 0,    // listed pixel pairs
