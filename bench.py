@@ -102,6 +102,61 @@ def copy_probe(eng, mib):
         % (nbytes, nbytes, ms, 2 * nbytes / (ms / 1e3) / 1e9))
 
 
+class StepRunner:
+    """Warm-up and timed steps under a wall-clock budget.
+
+    Runs up to `warmup` untimed steps (stopping early once they have used a fifth of the
+    budget) and then up to `steps` timed steps, stopping before a step that would end past
+    the budget; at least one timed step always runs.  `warmup_done` and `timings` hold what
+    actually ran, which is what the JSON line reports.
+    """
+
+    def __init__(self, warmup, steps, budget_s):
+        self.warmup = max(int(warmup), 0)
+        self.steps = max(int(steps), 1)
+        self.budget_s = float(budget_s)
+        self.warmup_done = 0
+        self.timings = []
+        self.results = []
+
+    def run(self, step, label="step"):
+        start = time.perf_counter()
+        last = 0.0
+        for w in range(self.warmup):
+            ts = time.perf_counter()
+            step()
+            last = time.perf_counter() - ts
+            self.warmup_done += 1
+            log("warmup %d: %.1f ms" % (w, last * 1e3))
+            if time.perf_counter() - start >= 0.2 * self.budget_s:
+                break
+        for k in range(self.steps):
+            elapsed = time.perf_counter() - start
+            expect = float(np.mean(self.timings)) if self.timings else last
+            if k > 0 and elapsed + expect > self.budget_s:
+                log("time budget of %.0f s reached after %d timed steps" % (self.budget_s, k))
+                break
+            ts = time.perf_counter()
+            out = step()
+            self.timings.append(time.perf_counter() - ts)
+            self.results.append(out)
+            log("%s %d: wall %.1f ms" % (label, k, self.timings[-1] * 1e3))
+        return self
+
+    @property
+    def steps_done(self):
+        return len(self.timings)
+
+    @property
+    def mean_s(self):
+        return float(np.mean(self.timings))
+
+    def record(self):
+        """The requested counts and the budget, for the JSON line's config."""
+        return {"warmup_requested": self.warmup, "steps_requested": self.steps,
+                "time_budget_s": self.budget_s}
+
+
 def sample_pairs(pairs, store, target_evaluated, seed=1):
     """Seeded random subset of forest pairs holding about target_evaluated pixel pairs."""
     nz = store.nonzero_weight_counts()
@@ -125,9 +180,12 @@ def main():
     parser.add_argument("--scale", type=float, default=1.0,
                         help="fraction of the sky area (and forests) of the workload")
     parser.add_argument("--seed", type=int, default=0)
-    parser.add_argument("--cpu-pairs", type=float, default=1.5e10,
+    parser.add_argument("--cpu-pairs", type=float, default=1.2e10,
                         help="evaluated pixel pairs in the CPU-baseline sample")
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    parser.add_argument("--time-budget", type=float, default=420.0,
+                        help="wall seconds for the warm-up and timed GPU steps together; "
+                             "the steps that fit are run and reported")
     parser.add_argument("--copy-probe", type=float, default=0.0,
                         help="MiB of a plain device copy to run before the steps")
     parser.add_argument("--traffic-bytes", type=float, default=-1.0,
@@ -208,20 +266,13 @@ def bench_xi(args):
     if args.copy_probe > 0:
         copy_probe(eng, args.copy_probe)
 
-    for w in range(args.warmup):
-        rep = eng.xi_execute(params)
-        log("warmup %d: kernel %.1f ms" % (w, rep.kernel_ms))
-    wall = []
-    kernel = []
-    report = None
-    for k in range(args.steps):
-        ts = time.perf_counter()
-        report = eng.xi_execute(params)   # returns after the stream has drained
-        wall.append(time.perf_counter() - ts)
-        kernel.append(report.kernel_ms)
-        log("step %d: wall %.1f ms, kernel %.1f ms" % (k, wall[-1] * 1e3, report.kernel_ms))
-    ms_per_step = 1e3 * float(np.mean(wall))
-    kernel_ms = float(np.mean(kernel))
+    # each step returns after the stream has drained
+    runner = StepRunner(args.warmup, args.steps, args.time_budget).run(
+        lambda: eng.xi_execute(params))
+    report = runner.results[-1]
+    ms_per_step = 1e3 * runner.mean_s
+    kernel_ms = float(np.mean([r.kernel_ms for r in runner.results]))
+    log("pair kernel: %.1f ms per step (HIP events)" % kernel_ms)
     value = evaluated / (ms_per_step / 1e3)
     rep = report.as_dict()
 
@@ -246,8 +297,8 @@ def bench_xi(args):
         "value": value,
         "unit": "pixel pairs/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": runner.steps_done,
+        "warmup": runner.warmup_done,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
@@ -273,54 +324,87 @@ def bench_xi(args):
             "z_min": survey.z_min, "ang_max_deg": float(np.degrees(ang_max)),
             "neighbour_search_s": neighbour_s, "upload_s": upload_s,
             "device": info["name"],
+            **runner.record(),
         },
         "roofline": roofline,
         "cpu_baseline": None,
     }
 
     if not args.no_cpu:
-        from oracle import cf_oracle as O
-        sample = sample_pairs(pairs, store, args.cpu_pairs)
-        sample_eval = sample.evaluated_pixel_pairs(store)
-        cores = cpu_cores()
-        log("cpu baseline: %d forest pairs, %.3e pixel pairs on %d threads"
-            % (sample.num_pairs, sample_eval, cores))
-        oparams = O.params_from_module(Globals)
-        tc = time.time()
-        out, stats = O.xi_rows(oparams, store, sample.pair_a, sample.pair_b, sample.pair_ang,
-                               sample.pair_flags, sample.pair_row, sample.num_rows,
-                               num_threads=cores)
-        cpu_s = time.time() - tc
-        cpu_rate = sample_eval / cpu_s
-        log("cpu baseline: %.1f s, %.3e pixel pairs/s" % (cpu_s, cpu_rate))
-        # GPU on the same sample: parity with the FP64 oracle
-        eng.set_xi_pairs(sample.pair_a, sample.pair_b, sample.pair_ang, sample.pair_flags,
-                         sample.pair_row, sample.num_rows, key="sample")
-        gres = eng.xi(params)
-        nb_mismatch = int(np.count_nonzero(gres.num_pairs != out[5]))
-        we_o = out[0]
-        mask = we_o > 0
-        xi_g = gres.xi[mask] / gres.weights[mask]
-        xi_o = out[1][mask] / we_o[mask]
-        rel = np.abs(xi_g - xi_o) / np.maximum(np.abs(xi_o), 1e-300)
-        result["cpu_baseline"] = {
-            "value": cpu_rate, "unit": "pixel pairs/s", "cores": cores, "kind": "port",
-            "sample": "%d random forest pairs (seed 1) of the same workload, %.3e pixel pairs, "
-                      "FP64 C oracle (scalar restatement of the reference loop), one OpenMP "
-                      "task per healpix row" % (sample.num_pairs, sample_eval),
-            "seconds": cpu_s,
-        }
-        result["parity_sample"] = {
-            "num_pairs_mismatches": nb_mismatch,
-            "accepted_pairs_gpu": int(gres.report["accepted_pairs"]),
-            "accepted_pairs_oracle": int(stats[0]),
-            "edge_pairs_gpu": int(gres.report["edge_pairs"]),
-            "edge_pairs_oracle": int(stats[1]),
-            "xi_max_rel_err": float(rel.max()) if rel.size else 0.0,
-            "xi_median_rel_err": float(np.median(rel)) if rel.size else 0.0,
-        }
-        result["speedup_vs_cpu"] = value / cpu_rate
+        try:
+            cpu_leg_xi(args, eng, params, store, pairs, value, result)
+        except Exception as exc:  # the JSON line is printed whatever the CPU legs do
+            log("cpu baseline failed: %r" % (exc,))
+            result["cpu_baseline_error"] = repr(exc)
     print(json.dumps(result))
+
+
+def cpu_leg_xi(args, eng, params, store, pairs, value, result):
+    """CPU baseline and parity sample of the xi bench.
+
+    The FP64 C oracle runs on a seeded random sample of the forest pairs, one OpenMP task
+    per healpix row, on every core this process may use; a second, smaller sample runs with
+    picca_cf's default process count, cpu_count() // 2 (py/picca/bin/picca_cf.py:340-341),
+    whatever the affinity allows.  The GPU then runs the first sample for parity.
+    """
+    from oracle import cf_oracle as O
+    sample = sample_pairs(pairs, store, args.cpu_pairs)
+    sample_eval = sample.evaluated_pixel_pairs(store)
+    cores = cpu_cores()
+    log("cpu baseline: %d forest pairs, %.3e pixel pairs on %d threads"
+        % (sample.num_pairs, sample_eval, cores))
+    oparams = O.params_from_module(Globals)
+    tc = time.time()
+    out, stats = O.xi_rows(oparams, store, sample.pair_a, sample.pair_b, sample.pair_ang,
+                           sample.pair_flags, sample.pair_row, sample.num_rows,
+                           num_threads=cores)
+    cpu_s = time.time() - tc
+    cpu_rate = sample_eval / cpu_s
+    log("cpu baseline: %.1f s, %.3e pixel pairs/s" % (cpu_s, cpu_rate))
+    result["cpu_baseline"] = {
+        "value": cpu_rate, "unit": "pixel pairs/s", "cores": cores, "kind": "port",
+        "sample": "%d random forest pairs (seed 1) of the same workload, %.3e pixel pairs, "
+                  "FP64 C oracle (scalar restatement of the reference loop), one OpenMP "
+                  "task per healpix row" % (sample.num_pairs, sample_eval),
+        "seconds": cpu_s,
+    }
+    result["speedup_vs_cpu"] = value / cpu_rate
+
+    # picca_cf's default worker count
+    default_threads = max(1, (os.cpu_count() or 2) // 2)
+    small = sample_pairs(pairs, store, args.cpu_pairs / 3, seed=2)
+    small_eval = small.evaluated_pixel_pairs(store)
+    tc = time.time()
+    O.xi_rows(oparams, store, small.pair_a, small.pair_b, small.pair_ang, small.pair_flags,
+              small.pair_row, small.num_rows, num_threads=default_threads)
+    small_s = time.time() - tc
+    log("cpu baseline at cpu_count()//2 = %d threads: %.1f s, %.3e pixel pairs/s"
+        % (default_threads, small_s, small_eval / small_s))
+    result["cpu_baseline"]["picca_default_nproc"] = {
+        "value": small_eval / small_s, "threads": default_threads,
+        "cores_available": cores,
+        "sample": "%d random forest pairs (seed 2), %.3e pixel pairs"
+                  % (small.num_pairs, small_eval)}
+
+    # GPU on the first sample: parity with the FP64 oracle
+    eng.set_xi_pairs(sample.pair_a, sample.pair_b, sample.pair_ang, sample.pair_flags,
+                     sample.pair_row, sample.num_rows, key="sample")
+    gres = eng.xi(params)
+    nb_mismatch = int(np.count_nonzero(gres.num_pairs != out[5]))
+    we_o = out[0]
+    mask = we_o > 0
+    xi_g = gres.xi[mask] / gres.weights[mask]
+    xi_o = out[1][mask] / we_o[mask]
+    rel = np.abs(xi_g - xi_o) / np.maximum(np.abs(xi_o), 1e-300)
+    result["parity_sample"] = {
+        "num_pairs_mismatches": nb_mismatch,
+        "accepted_pairs_gpu": int(gres.report["accepted_pairs"]),
+        "accepted_pairs_oracle": int(stats[0]),
+        "edge_pairs_gpu": int(gres.report["edge_pairs"]),
+        "edge_pairs_oracle": int(stats[1]),
+        "xi_max_rel_err": float(rel.max()) if rel.size else 0.0,
+        "xi_median_rel_err": float(np.median(rel)) if rel.size else 0.0,
+    }
 
 
 

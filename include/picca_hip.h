@@ -333,7 +333,8 @@ int picca_cov_subsample(picca_ctx *ctx, int64_t num_samples, int64_t num_bins,
 /* Counters of one Wick call (no counterpart in picca). */
 typedef struct picca_wick_report {
     int64_t in_range_pairs;  /* listed pixel pairs (sum of num_pairs_wick)           */
-    int64_t products;        /* (a, b) pixel-pair products evaluated                 */
+    int64_t products;        /* unordered pixel-pair products {a, b}, a != b: the
+                                iterations of the reference's index1 < index2 loop     */
     int64_t overflow_pairs;  /* in-range pairs whose bin rounds up to np (skipped;
                                 numba writes them out of bounds)                       */
     int64_t batches;         /* batches the pair list was split into                   */

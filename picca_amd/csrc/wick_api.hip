@@ -190,7 +190,10 @@ int picca_wick_t123(picca_ctx *ctx, const picca_cf_params *params, int64_t num_p
     }
     if (report) {
         report->in_range_pairs = (int64_t)h_stats[picca::WICK_STAT_IN_RANGE];
-        report->products = (int64_t)h_stats[picca::WICK_STAT_PRODUCTS];
+        // the row kernel walks every listed pair a against every b != a, so it sees each
+        // unordered pair {a, b} twice; the reference's loop (py/picca/cf.py:1573-1600)
+        // visits index1 < index2 only, one product per unordered pair
+        report->products = (int64_t)(h_stats[picca::WICK_STAT_PRODUCTS] / 2);
         report->overflow_pairs = (int64_t)h_stats[picca::WICK_STAT_OVERFLOW];
         report->batches = batches;
         report->kernel_ms = total_ms;
