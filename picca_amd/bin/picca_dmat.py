@@ -1,254 +1,60 @@
 #!/usr/bin/env python3
-"""Distortion matrix of the auto- and cross-correlation of delta fields on the MI355X engine.
+"""picca_dmat on the MI355X engine: the distortion matrix of the correlation of delta fields,
+written as picca's DMAT + ATTRI file.
 
-Command-line mirror of ``py/picca/bin/picca_dmat.py``: same flags and defaults, module
-globals (:353-394), reading (:396-465), the same split of the healpix list into
-``--nproc`` process lists (:470-476) -- each list seeds numpy's legacy stream with its first
-healpix before its rejection draws (``calc_dmat``, :21-38), so the kept pairs depend on
-``--nproc`` exactly as in picca -- the merge and normalisation (:493-519) and the ``DMAT``
-/ ``ATTRI`` output (:521-627).  The lists run one after the other, each as one batched
-GPU computation (``cf.compute_dmat``).
+Command line and products as ``py/picca/bin/picca_dmat.py``.  The healpix pixels are split
+into ``--nproc`` round-robin lists, each seeding numpy's legacy generator with its first
+pixel before its rejection draws, so the kept pairs are the reference's for the same
+``--nproc``; each list is one batched GPU computation (``cf.compute_dmat``).
 """
-import argparse
-import sys
-import time
-from multiprocessing import cpu_count
+from picca_amd import cf, constants, io
+from picca_amd import driver as D
 
-import numpy as np
-
-from picca_amd import cf, constants, io, utils
-from picca_amd.bin.picca_cf import reset_globals
-from picca_amd.fitsio_lite import FITSWriter
-from picca_amd.utils import userprint
+# the matrix is always referred to z = 2.25 (py/picca/bin/picca_dmat.py:377)
+MATRIX_Z_REF = 2.25
 
 
-def parse_args(cmdargs=None):
-    parser = argparse.ArgumentParser(
-        formatter_class=argparse.ArgumentDefaultsHelpFormatter,
-        description="Compute the distortion matrix of the auto and cross-correlation of "
-                    "delta fields")
-    parser.add_argument("--out", type=str, default=None, required=True, help="Output file name")
-    parser.add_argument("--in-dir", type=str, default=None, required=True,
-                        help="Directory to delta files")
-    parser.add_argument("--in-dir2", type=str, default=None, required=False,
-                        help="Directory to 2nd delta files")
-    parser.add_argument("--rp-min", type=float, default=0.0, required=False,
-                        help="Min r-parallel [h^-1 Mpc]")
-    parser.add_argument("--rp-max", type=float, default=200.0, required=False,
-                        help="Max r-parallel [h^-1 Mpc]")
-    parser.add_argument("--rt-max", type=float, default=200.0, required=False,
-                        help="Max r-transverse [h^-1 Mpc]")
-    parser.add_argument("--np", type=int, default=50, required=False,
-                        help="Number of r-parallel bins")
-    parser.add_argument("--nt", type=int, default=50, required=False,
-                        help="Number of r-transverse bins")
-    parser.add_argument("--rmu-binning", action="store_true",
-                        help="Estimate in r,mu binning (np: mu bins, nt: r bins)")
-    parser.add_argument("--coef-binning-model", type=int, default=1, required=False,
-                        help="Coefficient multiplying np and nt to get finner binning for "
-                             "the model")
-    parser.add_argument("--zerr-cut-deg", type=float, default=None, required=False,
-                        help="Angular separation below which the redshift-error cut applies")
-    parser.add_argument("--zerr-cut-kms", type=float, default=None, required=False,
-                        help="Velocity separation of the redshift-error cut [km/s]")
-    parser.add_argument("--z-min-sources", type=float, default=0.0, required=False,
-                        help="Limit the minimum redshift of the quasars")
-    parser.add_argument("--z-max-sources", type=float, default=10.0, required=False,
-                        help="Limit the maximum redshift of the quasars")
-    parser.add_argument("--z-min-pairs", type=float, default=None, required=False,
-                        help="Limit the minimum redshift of the pairs")
-    parser.add_argument("--z-max-pairs", type=float, default=None, required=False,
-                        help="Limit the maximum redshift of the pairs")
-    parser.add_argument("--lambda-abs", type=str, default="LYA", required=False,
-                        help="Name of the absorption in picca.constants defining the redshift")
-    parser.add_argument("--lambda-abs2", type=str, default=None, required=False,
-                        help="Name of the absorption of the 2nd delta field")
-    parser.add_argument("--z-evol", type=float, default=2.9, required=False,
-                        help="Exponent of the redshift evolution of the delta field")
-    parser.add_argument("--z-evol2", type=float, default=2.9, required=False,
-                        help="Exponent of the redshift evolution of the 2nd delta field")
-    parser.add_argument("--fid-Om", type=float, default=0.315, required=False,
-                        help="Omega_matter(z=0) of fiducial LambdaCDM cosmology")
-    parser.add_argument("--fid-Or", type=float, default=0.0, required=False,
-                        help="Omega_radiation(z=0) of fiducial LambdaCDM cosmology")
-    parser.add_argument("--fid-Ok", type=float, default=0.0, required=False,
-                        help="Omega_k(z=0) of fiducial LambdaCDM cosmology")
-    parser.add_argument("--fid-wl", type=float, default=-1.0, required=False,
-                        help="Equation of state of dark energy of fiducial LambdaCDM cosmology")
-    parser.add_argument("--distance-template", type=str, default=None, required=False,
-                        help="camb ini file for a distance template (not supported)")
-    parser.add_argument("--remove-same-half-plate-close-pairs", action="store_true",
-                        required=False,
-                        help="Reject pairs in the first bin in r-parallel from same half plate")
-    parser.add_argument("--rej", type=float, default=1.0, required=False,
-                        help="Fraction of rejected forest-forest pairs: -1=no rejection, "
-                             "1=all rejection")
-    parser.add_argument("--nside", type=int, default=16, required=False, help="Healpix nside")
-    parser.add_argument("--nproc", type=int, default=None, required=False,
-                        help="Number of process lists (fixes the rejection draws)")
-    parser.add_argument("--nspec", type=int, default=None, required=False,
-                        help="Maximum number of spectra to read")
-    parser.add_argument("--unfold-cf", action="store_true", required=False,
-                        help="rp can be positive or negative depending on the relative "
-                             "position between absorber1 and absorber2")
-    parser.add_argument("--rebin-factor", type=int, default=None, required=False,
-                        help="Rebin factor for deltas")
-    parser.add_argument("--in-attributes", type=str, default=None, required=False,
-                        help="Delta attributes file holding the continuum-fit order")
-    parser.add_argument("--no-redshift-evolution", action="store_true", required=False,
-                        help="Ignore the redshift evolution of the delta field")
-    parser.add_argument("--device", type=int, default=0, required=False,
-                        help="HIP device (engine option)")
-    return parser.parse_args(cmdargs)
-
-
-def calc_dmat(healpixs):
-    """One process list of picca_dmat (py/picca/bin/picca_dmat.py:21-38)."""
-    cf.fill_neighs(healpixs)
-    np.random.seed(healpixs[0])
-    return cf.compute_dmat(healpixs)
+def settings(opts, fields):
+    s = D.Settings(D.binning_settings(opts, model_coef=opts.coef_binning_model))
+    s.update(D.pair_cut_settings(opts))
+    s.update(z_ref=MATRIX_Z_REF, alpha=opts.z_evol, alpha2=opts.z_evol, reject=opts.rej,
+             lambda_abs=constants.ABSORBER_IGM[opts.lambda_abs],
+             remove_same_half_plate_close_pairs=opts.remove_same_half_plate_close_pairs,
+             redshift_evolution_in_distortion_matrix=not opts.no_redshift_evolution)
+    s.update(fields.settings(opts.z_evol2))
+    return s
 
 
 def main(cmdargs=None):
-    args = parse_args(cmdargs)
-    reset_globals()
-    cf.device = args.device
-    if args.nproc is None:
-        args.nproc = max(1, cpu_count() // 2)
-    cf.rmu_binning = args.rmu_binning
-    if cf.rmu_binning:
-        args.rp_min = 0
-        args.rp_max = 1
-    cf.r_par_max = args.rp_max
-    cf.r_par_min = args.rp_min
-    cf.r_trans_max = args.rt_max
-    cf.z_min_pairs = args.z_min_pairs
-    cf.z_max_pairs = args.z_max_pairs
-    cf.num_bins_r_par = args.np
-    cf.num_bins_r_trans = args.nt
-    cf.num_model_bins_r_par = args.np * args.coef_binning_model
-    cf.num_model_bins_r_trans = args.nt * args.coef_binning_model
-    cf.nside = args.nside
-    if args.no_redshift_evolution:
-        cf.redshift_evolution_in_distortion_matrix = False
-    cf.z_ref = 2.25
-    cf.alpha = args.z_evol
-    cf.alpha2 = args.z_evol
-    cf.reject = args.rej
-    cf.lambda_abs = constants.ABSORBER_IGM[args.lambda_abs]
-    cf.remove_same_half_plate_close_pairs = args.remove_same_half_plate_close_pairs
-    if (args.zerr_cut_deg is None) != (args.zerr_cut_kms is None):
-        raise ValueError("Options --zerr-cut-deg and --zerr-cut-kms must be specified together")
-    cf.zerr_cut_deg = args.zerr_cut_deg
-    cf.zerr_cut_kms = args.zerr_cut_kms
+    opts = D.parse("picca_dmat", cmdargs)
+    opts.z_ref = MATRIX_Z_REF
+    clock = D.Clock("picca_dmat")
+    cf.device = opts.device
+    D.process_count(opts)
+    D.pair_cut_settings(opts)
+    blinding = io.read_blinding(opts.in_dir)
+    fields = D.Fields(opts, D.cosmology(opts), opts.z_evol, opts.z_evol2)
+    settings(opts, fields).apply(cf)
+    clock.lap("read")
 
-    blinding = io.read_blinding(args.in_dir)
-    cosmo = constants.Cosmo(Om=args.fid_Om, Or=args.fid_Or, Ok=args.fid_Ok, wl=args.fid_wl,
-                            template=args.distance_template)
-    t0 = time.time()
-    data, num_data, z_min, z_max = io.read_deltas(
-        args.in_dir, cf.nside, cf.lambda_abs, cf.alpha, cf.z_ref, cosmo,
-        max_num_spec=args.nspec, nproc=args.nproc, rebin_factor=args.rebin_factor,
-        z_min_qso=args.z_min_sources, z_max_qso=args.z_max_sources,
-        delta_attributes=args.in_attributes)
-    del z_max
-    cf.data = data
-    cf.num_data = num_data
-    cf.ang_max = utils.compute_ang_max(cosmo, cf.r_trans_max, z_min)
-    userprint("")
-    userprint("done, npix = {}".format(len(data)))
-
-    if args.in_dir2 or args.lambda_abs2:
-        if args.lambda_abs2 or args.unfold_cf:
-            cf.x_correlation = True
-        cf.alpha2 = args.z_evol2
-        if args.in_dir2 is None:
-            args.in_dir2 = args.in_dir
-        if args.lambda_abs2:
-            cf.lambda_abs2 = constants.ABSORBER_IGM[args.lambda_abs2]
-        else:
-            cf.lambda_abs2 = cf.lambda_abs
-        data2, num_data2, z_min2, z_max2 = io.read_deltas(
-            args.in_dir2, cf.nside, cf.lambda_abs2, cf.alpha2, cf.z_ref, cosmo,
-            max_num_spec=args.nspec, nproc=args.nproc, rebin_factor=args.rebin_factor,
-            z_min_qso=args.z_min_sources, z_max_qso=args.z_max_sources,
-            delta_attributes=args.in_attributes)
-        del z_max2
-        cf.data2 = data2
-        cf.num_data2 = num_data2
-        cf.ang_max = utils.compute_ang_max(cosmo, cf.r_trans_max, z_min, z_min2)
-        userprint("")
-        userprint("done, npix = {}".format(len(data2)))
-
-    t1 = time.time()
-    userprint(f"picca_dmat.py - Time reading data: {(t1 - t0) / 60:.3f} minutes")
-
-    cpu_data = {}
-    for index, healpix in enumerate(sorted(data)):
-        cpu_data.setdefault(index % args.nproc, []).append(healpix)
-    dmat_data = [calc_dmat(lst) for lst in sorted(cpu_data.values())]
-    t2 = time.time()
-    userprint(f"picca_dmat.py - Time computing distortion matrix: {(t2 - t1) / 60:.3f} minutes")
-
-    weights_dmat = np.array([item[0] for item in dmat_data]).sum(axis=0)
-    dmat = np.array([item[1] for item in dmat_data]).sum(axis=0)
-    r_par = np.array([item[2] for item in dmat_data]).sum(axis=0)
-    r_trans = np.array([item[3] for item in dmat_data]).sum(axis=0)
-    zeff = np.array([item[4] for item in dmat_data]).sum(axis=0)
-    weights = np.array([item[5] for item in dmat_data]).sum(axis=0)
-    num_pairs = np.array([item[6] for item in dmat_data]).sum(axis=0)
-    num_pairs_used = np.array([item[7] for item in dmat_data]).sum(axis=0)
-
-    w = weights > 0.0
-    r_par[w] /= weights[w]
-    r_trans[w] /= weights[w]
-    zeff[w] /= weights[w]
-    mean_zeff = np.mean(zeff[w])
-    w = weights_dmat > 0
-    dmat[w] /= weights_dmat[w, None]
+    lists = D.process_lists(cf.data, opts.nproc)
+    parts = D.seeded_lists(cf, lists, cf.compute_dmat)
+    clock.lap("matrix")
+    weights_dmat, dmat, r_par, r_trans, z, weights, npall, npused = D.add_parts(parts)
+    mean_z = D.normalise_matrix(weights_dmat, dmat, r_par, r_trans, z, weights)
     if cf.redshift_evolution_in_distortion_matrix:
-        zeff[:] = mean_zeff
-        zfac = ((1 + cf.z_ref) / (1 + mean_zeff)) ** ((cf.alpha - 1) + (cf.alpha2 - 1))
-        dmat *= zfac
+        D.redshift_rescale(dmat, z, mean_z, cf.z_ref, cf.alpha, cf.alpha2)
 
-    header = [
-        {"name": "RPMIN", "value": cf.r_par_min, "comment": "Minimum r-parallel [h^-1 Mpc]"},
-        {"name": "RPMAX", "value": cf.r_par_max, "comment": "Maximum r-parallel [h^-1 Mpc]"},
-        {"name": "RTMAX", "value": cf.r_trans_max, "comment": "Maximum r-transverse [h^-1 Mpc]"},
-        {"name": "NP", "value": cf.num_bins_r_par, "comment": "Number of bins in r-parallel"},
-        {"name": "NT", "value": cf.num_bins_r_trans, "comment": "Number of bins in r-transverse"},
-        {"name": "COEFMOD", "value": args.coef_binning_model,
-         "comment": "Coefficient for model binning"},
-        {"name": "ZMIN", "value": cf.z_min_pairs, "comment": "Minimum redshift of pairs"},
-        {"name": "ZMAX", "value": cf.z_max_pairs, "comment": "Maximum redshift of pairs"},
-        {"name": "REJ", "value": cf.reject, "comment": "Rejection factor"},
-        {"name": "NPALL", "value": int(num_pairs), "comment": "Number of pairs"},
-        {"name": "NPUSED", "value": int(num_pairs_used), "comment": "Number of used pairs"},
-        {"name": "OMEGAM", "value": args.fid_Om,
-         "comment": "Omega_matter(z=0) of fiducial LambdaCDM cosmology"},
-        {"name": "OMEGAR", "value": args.fid_Or,
-         "comment": "Omega_radiation(z=0) of fiducial LambdaCDM cosmology"},
-        {"name": "OMEGAK", "value": args.fid_Ok,
-         "comment": "Omega_k(z=0) of fiducial LambdaCDM cosmology"},
-        {"name": "WL", "value": args.fid_wl,
-         "comment": "Equation of state of dark energy of fiducial LambdaCDM cosmology"},
-        {"name": "BLINDING", "value": blinding,
-         "comment": "String specifying the blinding strategy"},
-        {"name": "RMU_BIN", "value": cf.rmu_binning, "comment": "True if binned in r, mu"},
-    ]
-    dmat_name = "DM" if blinding == "none" else "DM_BLIND"
-    results = FITSWriter(args.out)
-    results.write([weights_dmat, dmat], names=["WDM", dmat_name],
-                  comment=["Sum of weight", "Distortion matrix"], units=["", ""],
-                  header=header, extname="DMAT")
-    results.write([r_par, r_trans, zeff], names=["RP", "RT", "Z"],
-                  comment=["R-parallel", "R-transverse", "Redshift"],
-                  units=["h^-1 Mpc", "h^-1 Mpc", ""], extname="ATTRI")
-    results.close()
-    t3 = time.time()
-    userprint(f"picca_dmat.py - Time total : {(t3 - t0) / 60:.3f} minutes")
+    cards = D.header(RPMIN=cf.r_par_min, RPMAX=cf.r_par_max, RTMAX=cf.r_trans_max,
+                     NP=cf.num_bins_r_par, NT=cf.num_bins_r_trans,
+                     COEFMOD=opts.coef_binning_model, ZMIN=cf.z_min_pairs,
+                     ZMAX=cf.z_max_pairs, REJ=cf.reject, NPALL=int(npall),
+                     NPUSED=int(npused))
+    cards += D.cosmo_cards(opts) + D.header(BLINDING=blinding, RMU_BIN=cf.rmu_binning)
+    D.matrix_products(opts.out, (weights_dmat, dmat, r_par, r_trans, z), cards, blinding)
+    clock.total()
     return cf.last_report
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:])
+    D.main_entry(main)

@@ -279,7 +279,8 @@ def build_xi_pairs(healpixs):
     pair_a = src[s_of]
     pair_b = cat.target_index[nl.target]
     if remove_same_half_plate_close_pairs:
-        for k in np.unique(pair_a):
+        # either side of a pair with a combined-reobservation id (py/picca/cf.py:171-179)
+        for k in np.unique(np.concatenate([pair_a, pair_b])):
             if isinstance(cat.deltas[k].fiberid, str):
                 raise RuntimeError(
                     "Trying to figure out if two spectra come from the same half plate "
