@@ -207,6 +207,9 @@ def main():
                         help="wick: forest pairs of the workload")
     parser.add_argument("--wick-cpu-pairs", type=int, default=3,
                         help="wick: forest pairs of the CPU-baseline sample")
+    parser.add_argument("--schedule", default="auto", choices=("auto", "pairs", "runs"),
+                        help="xi: schedule of the pair kernel (auto: run walk where a bin "
+                             "holds enough pixels, else the pair loop)")
     parser.add_argument("--spectra", type=int, default=200000,
                         help="continuum: spectra fitted (and as many more for training)")
     args = parser.parse_args()
@@ -254,6 +257,9 @@ def bench_xi(args):
     params = params_from_globals(Globals)
     eng = get_engine(0)
     info = device_info(0)
+    from picca_amd import _native
+    eng.set_xi_schedule({"auto": _native.XI_SCHEDULE_AUTO, "pairs": _native.XI_SCHEDULE_PAIRS,
+                         "runs": _native.XI_SCHEDULE_RUNS}[args.schedule])
     t2 = time.time()
     eng.set_forests(store, key="bench")
     eng.set_xi_pairs(pairs.pair_a, pairs.pair_b, pairs.pair_ang, pairs.pair_flags,
@@ -318,6 +324,10 @@ def bench_xi(args):
             "kernel_wave_steps": rep["wave_steps"],
             "kernel_tasks": rep["tasks"],
             "kernel_empty_tasks": rep["empty_tasks"],
+            "schedule": args.schedule,
+            "kernel_walk_tasks": rep["walk_tasks"],
+            "kernel_runs": rep["runs"],
+            "kernel_probes": rep["probes"],
             "edge_pairs": rep["edge_pairs"],
             "overflow_pairs": rep["overflow_pairs"],
             "bins": "50x50", "r_par_max": 200.0, "r_trans_max": 200.0, "nside": survey.nside,

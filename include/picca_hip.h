@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define PICCA_HIP_ABI_VERSION 2
+#define PICCA_HIP_ABI_VERSION 3
 
 #define PICCA_OK 0
 #define PICCA_E_VALUE (-1)   /* invalid argument: picca raises ValueError          */
@@ -91,6 +91,9 @@ typedef struct picca_xi_report {
     int64_t empty_tasks;     /* work items whose pruned pixel window was empty      */
     double kernel_ms;        /* device time of the pair kernel (HIP events)         */
     int64_t launches;        /* pair-kernel launches inside kernel_ms               */
+    int64_t runs;            /* run walk: runs of one bin summed from prefix sums    */
+    int64_t probes;          /* run walk: pixels probed to find the runs' ends       */
+    int64_t walk_tasks;      /* work items taken by the run walk                     */
 } picca_xi_report;
 
 typedef struct picca_ctx picca_ctx;
@@ -133,6 +136,19 @@ int picca_xi_set_pairs(picca_ctx *ctx, int64_t num_pairs, const int32_t *pair_a,
 
 /* Run the pair kernel over the resident pairs; results stay on the device. */
 int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_report *report);
+
+/*
+ * Schedule of the pair kernel's specialised path (standard binning, auto-correlation,
+ * r_par_min == 0, no cuts).  RUNS walks each pixel of delta1 through runs of pixels of
+ * delta2 that share one bin, summed from per-forest prefix sums, wherever delta2's pixels
+ * increase strictly and are spaced well above the rounding margins; PAIRS evaluates every
+ * pixel pair.  AUTO (the default) is RUNS.  Results are identical up to FP64 summation
+ * order; no counterpart in picca.
+ */
+#define PICCA_XI_SCHEDULE_AUTO 0
+#define PICCA_XI_SCHEDULE_PAIRS 1
+#define PICCA_XI_SCHEDULE_RUNS 2
+int picca_xi_set_schedule(picca_ctx *ctx, int32_t schedule);
 
 /*
  * Copy the un-normalised sums of the last execution to the host, each [num_rows * np*nt]:

@@ -19,6 +19,11 @@ PICCA_E_HIP = -4
 PICCA_E_NOMEM = -5
 
 PICCA_PAIR_SAME_HALF_PLATE = 1
+
+# schedules of the pair kernel's specialised path (picca_xi_set_schedule)
+XI_SCHEDULE_AUTO = 0
+XI_SCHEDULE_PAIRS = 1
+XI_SCHEDULE_RUNS = 2
 PICCA_PAIR_XCF_COUNT = 2
 PICCA_METAL_AUTO = 0
 PICCA_METAL_CROSS = 1
@@ -68,6 +73,9 @@ class XiReport(ctypes.Structure):
         ("empty_tasks", ctypes.c_int64),
         ("kernel_ms", ctypes.c_double),
         ("launches", ctypes.c_int64),
+        ("runs", ctypes.c_int64),
+        ("probes", ctypes.c_int64),
+        ("walk_tasks", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -124,6 +132,7 @@ _PROTOTYPES = {
     "picca_xi_set_pairs": [_P, _I64, _P, _P, _P, _P, _P, _I32],
     "picca_xi_execute": [_P, _P, _P],
     "picca_xi_download": [_P, _P, _P, _P, _P, _P, _P],
+    "picca_xi_set_schedule": [_P, _I32],
     "picca_dmat_run": [_P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "picca_diag_copy": [_P, _I64, _P],
     "picca_cov_subsample": [_P, _I64, _I64, _P, _P, _P, _P],
@@ -150,7 +159,7 @@ _load_error = None
 
 
 # PICCA_HIP_ABI_VERSION of include/picca_hip.h that these structures mirror
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 
 def exported_symbols():

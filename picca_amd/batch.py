@@ -8,6 +8,7 @@ engine.
 """
 import numpy as np
 
+from . import _native
 from . import engine as _engine
 from .neighbours import find_neighbours
 
@@ -63,10 +64,19 @@ def xi_pairs(store, healpix, ang_max, target_store_index=None, target_healpix=No
     return XiPairs(pair_a, pair_b, nl.angle, flags, row_of_src[s_of], uniq)
 
 
-def run_xi(store, pairs, params, device=0, store_key=None, pairs_key=None):
-    """Upload (if needed) and run; returns an engine.XiResult with [rows, nb] sums."""
+def run_xi(store, pairs, params, device=0, store_key=None, pairs_key=None, schedule=None):
+    """Upload (if needed) and run; returns an engine.XiResult with [rows, nb] sums.
+
+    schedule: None keeps the engine's (AUTO unless set), else one of
+    _native.XI_SCHEDULE_AUTO / _PAIRS / _RUNS."""
     eng = _engine.get_engine(device)
     eng.set_forests(store, key=store_key)
     eng.set_xi_pairs(pairs.pair_a, pairs.pair_b, pairs.pair_ang, pairs.pair_flags,
                      pairs.pair_row, pairs.num_rows, key=pairs_key)
-    return eng.xi(params)
+    if schedule is None:
+        return eng.xi(params)
+    eng.set_xi_schedule(schedule)
+    try:
+        return eng.xi(params)
+    finally:
+        eng.set_xi_schedule(_native.XI_SCHEDULE_AUTO)

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "context.h"
+#include "xi_const.h"
 
 namespace picca_host {
 
@@ -29,54 +30,8 @@ bool forest_monotone(const double *x, int64_t n) {
 }
 
 int build_xi_const(const picca_cf_params *p, picca::XiConst &C, int &mode) {
-    if (!p) return fail(PICCA_E_VALUE, "params is NULL");
-    if (p->num_bins_r_par <= 0 || p->num_bins_r_trans <= 0)
-        return fail(PICCA_E_VALUE, "num_bins_r_par and num_bins_r_trans must be positive");
-    if (!(p->r_par_max > p->r_par_min))
-        return fail(PICCA_E_VALUE, "r_par_max must be larger than r_par_min");
-    if (!(p->r_trans_max > 0))
-        return fail(PICCA_E_VALUE, "r_trans_max must be positive");
-    if (!(p->edge_tol >= 0)) return fail(PICCA_E_VALUE, "edge_tol must be >= 0");
-    if (p->has_zerr_cut && !(p->zerr_cut_kms == p->zerr_cut_kms))
-        return fail(PICCA_E_VALUE, "zerr_cut_kms must be given with zerr_cut_deg");
-    std::memset(&C, 0, sizeof(C));
-    C.np = p->num_bins_r_par;
-    C.nt = p->num_bins_r_trans;
-    C.nb = C.np * C.nt;
-    C.ntp = C.nt + 1;
-    C.rpmin = p->r_par_min;
-    C.rpmax = p->r_par_max;
-    C.rtmax = p->r_trans_max;
-    C.span_p = p->r_par_max - p->r_par_min;
-    C.np_d = (double)C.np;
-    C.nt_d = (double)C.nt;
-    C.nb_d = (double)C.nb;
-    C.ntp_d = (double)C.ntp;
-    C.inv_p = C.np_d / C.span_p;
-    C.inv_t = C.nt_d / C.rtmax;
-    C.width_p = C.span_p / C.np;
-    C.width_t = C.rtmax / C.nt;
-    C.edge_tol = p->edge_tol;
-    // |t_fast - t_exact| <= 4 ulp(t) << 1e-12 for t < 2^20; the margin also absorbs the
-    // rounding of frac * width in the edge test.
-    C.fast_p = p->edge_tol / C.width_p * (1.0 + 1e-6) + 1e-12;
-    C.fast_t = p->edge_tol / C.width_t * (1.0 + 1e-6) + 1e-12;
-    C.half_p = 0.5 - C.fast_p;
-    C.half_t = 0.5 - C.fast_t;
-    C.zmin = p->z_min_pairs;
-    C.zmax = p->z_max_pairs;
-    C.zerr_ang = p->zerr_cut_deg * M_PI / 180.0;
-    C.zerr_kms = p->zerr_cut_kms;
-    int flags = 0;
-    if (p->has_z_min_pairs) flags |= picca::XI_HAS_ZMIN;
-    if (p->has_z_max_pairs) flags |= picca::XI_HAS_ZMAX;
-    if (p->has_zerr_cut) flags |= picca::XI_HAS_ZERR;
-    if (p->remove_same_half_plate_close_pairs) flags |= picca::XI_REMOVE_SHP;
-    mode = p->ang_correlation ? picca::MODE_ANGULAR
-                              : (p->rmu_binning ? picca::MODE_RMU : picca::MODE_STANDARD);
-    if (mode == picca::MODE_STANDARD) flags |= picca::XI_PRUNE;
-    C.flags = flags;
-    return PICCA_OK;
+    const char *err = picca::fill_xi_const(p, C, mode);
+    return err ? fail(PICCA_E_VALUE, err) : PICCA_OK;
 }
 
 }  // namespace
@@ -227,6 +182,30 @@ int picca_set_forests(picca_ctx *ctx, int64_t num_forests, const int64_t *offset
         PICCA_HIP_CHECK(ctx->pix.upload_padded(records.data(), (size_t)npix, pad, ctx->stream));
         PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     }
+    {
+        // run walk of the xi kernel: (r, d) pairs, prefix sums (L + 1 entries per forest)
+        // and the smallest pixel steps per forest
+        std::vector<double2> rd((size_t)std::max<int64_t>(npix, 1));
+        std::vector<picca::PrefixRecord> pre((size_t)(npix + num_forests));
+        std::vector<double2> spc((size_t)std::max<int64_t>(num_forests, 1));
+        parallel_for(npix, [&](int64_t lo, int64_t hi) {
+            for (int64_t k = lo; k < hi; ++k) rd[k] = make_double2(r_comov[k], dist_m[k]);
+        });
+        parallel_for(num_forests, [&](int64_t lo, int64_t hi) {
+            for (int64_t k = lo; k < hi; ++k) {
+                const int64_t o = offsets[k], n = offsets[k + 1] - offsets[k];
+                double sr, sd;
+                picca::build_walk_forest(r_comov + o, dist_m + o, z + o, weights + o,
+                                         dw.data() + o, n, pre.data() + o + k, sr, sd);
+                spc[k] = make_double2(sr, sd);
+            }
+        });
+        PICCA_HIP_CHECK(ctx->rd.upload_padded(rd.data(), (size_t)npix, pad, ctx->stream));
+        PICCA_HIP_CHECK(ctx->pre.upload_padded(pre.data(), (size_t)(npix + num_forests), pad,
+                                               ctx->stream));
+        PICCA_HIP_CHECK(ctx->spc.upload(spc.data(), (size_t)num_forests, ctx->stream));
+        PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
     PICCA_HIP_CHECK(ctx->zq.upload(z_qso, (size_t)num_forests, ctx->stream));
     PICCA_HIP_CHECK(ctx->mono.upload(mono.data(), (size_t)num_forests, ctx->stream));
     ctx->has_log_lambda = log_lambda != nullptr;
@@ -341,28 +320,13 @@ int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_rep
     int mode = 0;
     int status = build_xi_const(params, C, mode);
     if (status != PICCA_OK) return status;
-    // The fast loop takes the bin coordinates as |r1 - r2| * (cos * np / span) and
-    // (d1 + d2) * (sin * nt / r_trans_max) instead of the reference's
-    // ((r1 - r2) * cos - r_par_min) / span * np and (d1 + d2) * sin / r_trans_max * nt; both
-    // differ from the exact value by a few ulp of the coordinate.  The margins around every
-    // bin edge and range limit are widened by a bound of that difference (in bin units), so
-    // that every pair whose bin or range test could differ is re-evaluated exactly.
     {
-        const double u = std::ldexp(1.0, -53);
-        const double err_p = 16.0 * u * ctx->max_abs_r * C.inv_p + 8.0 * u * (C.np_d + 1.0);
-        const double err_t = 16.0 * u * ctx->max_abs_d * C.inv_t + 8.0 * u * (C.nt_d + 1.0);
-        C.fast_p += err_p;
-        C.fast_t += err_t;
-        C.half_p = 0.5 - C.fast_p;
-        C.half_t = 0.5 - C.fast_t;
-        if (!(C.half_p > 0.25 && C.half_t > 0.25))
-            return fail(PICCA_E_VALUE, "edge_tol is too large compared with the bin width");
+        const char *err = picca::widen_xi_margins(C, ctx->max_abs_r, ctx->max_abs_d);
+        if (err) return fail(PICCA_E_VALUE, err);
     }
     // the specialised kernel: standard binning, auto-correlation, r_par_min == 0, no cuts
-    const bool general = mode != picca::MODE_STANDARD || params->x_correlation ||
-                         params->r_par_min != 0.0 || params->has_z_min_pairs ||
-                         params->has_z_max_pairs || params->has_zerr_cut ||
-                         params->remove_same_half_plate_close_pairs;
+    const bool general = !picca::xi_specialised(params, mode);
+    picca::apply_xi_schedule(C, ctx->xi_schedule);
     // LDS: the histogram in its padded layout ((np + 1) x (nt + 1) bins of 48 B) when it
     // fits, plus one staging tile per wave for the loops that stage pixels (the fast loop
     // of the specialised kernel stages none)
@@ -405,7 +369,8 @@ int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_rep
     PICCA_HIP_CHECK(picca::launch_zero_u64(ctx->stats.ptr, picca::XI_NSTATS, ctx->stream));
 
     picca::ForestView F{ctx->pix.ptr, ctx->z.ptr, ctx->r.ptr, ctx->d.ptr, ctx->w.ptr, ctx->dw.ptr,
-                        ctx->ll.ptr, ctx->off.ptr, ctx->zq.ptr, ctx->order.ptr, ctx->mono.ptr};
+                        ctx->ll.ptr, ctx->off.ptr, ctx->zq.ptr, ctx->order.ptr, ctx->mono.ptr,
+                        ctx->rd.ptr, ctx->pre.ptr, ctx->spc.ptr};
     picca::PairView P{ctx->pa.ptr, ctx->pb.ptr, ctx->pang.ptr, ctx->pcs.ptr, ctx->pflags.ptr};
     const int64_t ngroups = (int64_t)ctx->h_group_row.size();
     picca::GroupView G{ctx->gbegin.ptr, ctx->grow.ptr, ngroups};
@@ -434,9 +399,21 @@ int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_rep
         report->wave_steps = (int64_t)stats[picca::XI_STAT_STEPS];
         report->tasks = (int64_t)stats[picca::XI_STAT_TASKS];
         report->empty_tasks = (int64_t)stats[picca::XI_STAT_EMPTY_TASKS];
+        report->runs = (int64_t)stats[picca::XI_STAT_RUNS];
+        report->probes = (int64_t)stats[picca::XI_STAT_PROBES];
+        report->walk_tasks = (int64_t)stats[picca::XI_STAT_WALK_TASKS];
         report->kernel_ms = (double)ms;
         report->launches = ngroups > 0 ? 1 : 0;
     }
+    return PICCA_OK;
+}
+
+int picca_xi_set_schedule(picca_ctx *ctx, int32_t schedule) {
+    if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
+    if (schedule != PICCA_XI_SCHEDULE_AUTO && schedule != PICCA_XI_SCHEDULE_PAIRS &&
+        schedule != PICCA_XI_SCHEDULE_RUNS)
+        return fail(PICCA_E_VALUE, "unknown xi schedule");
+    ctx->xi_schedule = schedule;
     return PICCA_OK;
 }
 
@@ -696,7 +673,8 @@ int picca_xcf_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_re
                                            nout * (int64_t)(sizeof(picca::XiBin) / 8), ctx->stream));
     PICCA_HIP_CHECK(picca::launch_zero_u64(ctx->stats.ptr, picca::XI_NSTATS, ctx->stream));
     picca::ForestView F{ctx->pix.ptr, ctx->z.ptr, ctx->r.ptr, ctx->d.ptr, ctx->w.ptr, ctx->dw.ptr,
-                        ctx->ll.ptr, ctx->off.ptr, ctx->zq.ptr, ctx->order.ptr, ctx->mono.ptr};
+                        ctx->ll.ptr, ctx->off.ptr, ctx->zq.ptr, ctx->order.ptr, ctx->mono.ptr,
+                        ctx->rd.ptr, ctx->pre.ptr, ctx->spc.ptr};
     picca::XiOut O{ctx->o_bins.ptr, ctx->queue.ptr, ctx->stats.ptr};
     PICCA_HIP_CHECK(hipEventRecord(ctx->ev_start, ctx->stream));
     int64_t launches = 0;

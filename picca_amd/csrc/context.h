@@ -12,6 +12,7 @@
 
 #include "../../include/picca_hip.h"
 #include "engine.h"
+#include "xi_walk.h"
 
 namespace picca_host {
 
@@ -119,6 +120,10 @@ struct picca_ctx {
     DeviceBuffer<int64_t> off;
     DeviceBuffer<int32_t> order;
     DeviceBuffer<uint8_t> mono;
+    // run walk of the xi kernel (xi_walk.h)
+    DeviceBuffer<double2> rd, spc;
+    DeviceBuffer<picca::PrefixRecord> pre;
+    int32_t xi_schedule = 0;   // PICCA_XI_SCHEDULE_*
 
     // xi pairs
     int64_t num_pairs = 0;

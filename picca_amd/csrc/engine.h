@@ -19,6 +19,7 @@ enum : int {
     XI_HAS_ZERR = 4,
     XI_REMOVE_SHP = 8,
     XI_PRUNE = 16,
+    XI_WALK = 32,   // run walk allowed for eligible tasks of the specialised kernel
 };
 
 enum : int { PAIR_SAME_HALF_PLATE = 1, PAIR_XCF_COUNT = 2 };
@@ -31,7 +32,10 @@ enum : int {
     XI_STAT_ACCEPTED = 4,
     XI_STAT_TASKS = 5,
     XI_STAT_EMPTY_TASKS = 6,
-    XI_NSTATS = 7,
+    XI_STAT_RUNS = 7,
+    XI_STAT_PROBES = 8,
+    XI_STAT_WALK_TASKS = 9,
+    XI_NSTATS = 10,
 };
 
 // Launch constants derived on the host from picca_cf_params.
@@ -56,6 +60,9 @@ struct XiConst {
     // r_trans row window of the cross-correlation kernel's LDS histogram: rows
     // [win_t0, win_t0 + win_rows) of the output, padded to (np + 2) x (win_rows + 2) bins
     int32_t win_t0, win_rows, win_last, win_pad_;
+    // run walk of the xi kernel: taken for a task whose delta2 holds at least this many
+    // pixels per r_par bin width (pixel pairs per run), where it beats the pair loop
+    double walk_min_q;
 };
 
 // One pixel as the pair kernel streams it through the scalar cache.
@@ -66,6 +73,8 @@ struct PixelRecord {
     double w;   // weight
     double dw;  // delta * weight
 };
+
+struct PrefixRecord;  // xi_walk.h
 
 struct ForestView {
     const PixelRecord *__restrict__ pix;  // the same pixels, one 40-byte record each
@@ -79,6 +88,11 @@ struct ForestView {
     const double *__restrict__ zq;
     const int32_t *__restrict__ order;
     const uint8_t *__restrict__ mono; // bit0 r_comov non-decreasing, bit1 dist_m non-decreasing
+    const double2 *__restrict__ rd;   // (r_comov, dist_m) per pixel
+    const PrefixRecord *__restrict__ pre;       // run-walk prefix sums, forest b at off[b] + b
+    const double2 *__restrict__ spc;  // per forest: smallest step of r_comov and of dist_m
+                                      // between consecutive pixels, -1 unless both increase
+                                      // strictly
 };
 
 struct PairView {

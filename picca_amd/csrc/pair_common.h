@@ -95,6 +95,8 @@ struct LaneCounters {
     uint64_t accept;
     uint32_t edge;
     uint32_t overflow;
+    uint64_t runs;     // run walk: runs and probes of the lane
+    uint64_t probes;
 };
 
 // Adds one accepted pair to the lane's run, flushing the run when the bin changes.  The

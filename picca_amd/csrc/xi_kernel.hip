@@ -26,7 +26,12 @@
 //     full).  The skipped pixel pairs are provably rejected by the reference's range test;
 //   * a lane accumulates consecutive pairs that land in the same bin in registers
 //     (r_par and r_trans move monotonically with j) and adds the run to the histogram
-//     with six LDS atomics when its bin changes.
+//     with six LDS atomics when its bin changes;
+//   * run walk (xi_walk.h), the specialised kernel's schedule wherever delta2's pixels
+//     increase strictly and a bin holds at least a few of them: each lane walks its own
+//     runs of delta2 pixels with one bin, finds a run's end with about two probes and adds
+//     the run from per-forest prefix sums -- one flush per run instead of one evaluation
+//     per pair.
 //
 // Arithmetic: r_par, r_trans and the bin index follow the reference's FP64 operations
 // (built with -ffp-contract=off).  The bin index is first taken with a reciprocal
@@ -42,6 +47,7 @@
 #include "device_util.h"
 #include "engine.h"
 #include "pair_common.h"
+#include "xi_walk.h"
 
 namespace picca {
 
@@ -190,6 +196,54 @@ __device__ __forceinline__ double min_f64(double a, double b) {
 
 typedef __attribute__((address_space(3))) unsigned long long LocalU64;
 
+// Memory of the run walk on the device: delta2's pixels and prefix sums in HBM (read
+// per lane), the LDS histogram of the output row, the LDS statistics.
+struct DeviceWalkMem {
+    const double2 *__restrict__ rdp;
+    const PrefixRecord *__restrict__ pre;
+    const double *__restrict__ wp;
+    const double *__restrict__ zp;
+    const double *__restrict__ dwp;
+    LocalBin *hist;
+    LocalU64 *stats;
+    uint32_t probes, runs;
+    double cover;
+
+    __device__ void rd(int p, double &r, double &d) const {
+        const double2 v = rdp[p];
+        r = v.x;
+        d = v.y;
+    }
+    __device__ PrefixRecord prefix(int p) const { return pre[p]; }
+    __device__ double w(int p) const { return wp[p]; }
+    __device__ double z(int p) const { return zp[p]; }
+    __device__ double dw(int p) const { return dwp[p]; }
+    __device__ void add(int bin, double we, double xi, double rp, double rt, double z,
+                        uint32_t n) {
+        LocalBin *b = (LocalBin *)((LocalByte *)hist +
+                                   __umul24((uint32_t)bin, (uint32_t)sizeof(LdsBin)));
+        __hip_atomic_fetch_add(&b->we, we, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&b->xi, xi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&b->rp, rp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&b->rt, rt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&b->z, z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&b->nb, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __device__ void count_edge() {
+        __hip_atomic_fetch_add(&stats[XI_STAT_EDGE], 1ull, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __device__ void count_overflow() {
+        __hip_atomic_fetch_add(&stats[XI_STAT_OVERFLOW], 1ull, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __device__ void count_probe() { ++probes; }
+    __device__ void count_run(double n) {
+        ++runs;
+        cover += n;
+    }
+};
+
 // One task: pixels [64*chunk, 64*chunk + 64) of delta1 against the pruned window of delta2.
 //   GENERAL = false: standard binning, auto-correlation, r_par_min == 0, no z / zerr /
 //   same-half-plate cuts (the cuts of the reference that cannot trigger are compiled out).
@@ -199,7 +253,7 @@ template <int MODE, bool XCORR, bool GENERAL, bool LDS_HIST, typename Bin>
 __device__ void xi_task(const XiConst &C, const XiConst &C_lds, const ForestView &F,
                         const PairView &P, int64_t p, int chunk, int lane, Bin *hist,
                         TilePixel *tile, TileExact *tile_x, LaneCounters &cnt,
-                        LocalU64 *lds_stats, uint64_t &n_steps) {
+                        LocalU64 *lds_stats, uint64_t &n_steps, uint64_t &n_walk) {
     const int a = uniform_int(P.a[p]);
     const int b = uniform_int(P.b[p]);
     const int64_t oa = uniform_i64(F.off[a]);
@@ -250,8 +304,10 @@ __device__ void xi_task(const XiConst &C, const XiConst &C_lds, const ForestView
     const double *__restrict__ Bdw = F.dw + ob;
 
     int j_lo = 0, j_hi = lb;
+    bool pruned = false;
     if (MODE == MODE_STANDARD && (C.flags & XI_PRUNE) && cos_half > 0 &&
         (uniform_int(F.mono[b]) & 3) == 3) {
+        pruned = true;
         const double r_min = wave_min(active ? r1 : INFINITY);
         const double r_max = wave_max(active ? r1 : -INFINITY);
         const double d_min = wave_min(active ? d1 : INFINITY);
@@ -274,6 +330,46 @@ __device__ void xi_task(const XiConst &C, const XiConst &C_lds, const ForestView
     j_hi = uniform_int(j_hi);
 
     if constexpr (!GENERAL && LDS_HIST) {
+        if ((C.flags & XI_WALK) && pruned) {
+            // Run walk (xi_walk.h) when delta2's pixels are strictly increasing in r_comov
+            // and dist_m and consecutive pixels lie more than four rounding margins apart
+            // in both bin coordinates; otherwise the pair loop below.
+            const double spc_r = uniform_f64(F.spc[b].x);
+            const double spc_d = uniform_f64(F.spc[b].y);
+            const double cp = cos_half * C.inv_p;
+            const double st = sin_half * C.inv_t;
+            double q0 = 1.0;
+            if (j_hi - j_lo >= 2) {
+                const double span = (uniform_f64(F.rd[ob + j_hi - 1].x) -
+                                     uniform_f64(F.rd[ob + j_lo].x)) * cp;
+                if (span > 0) q0 = (double)(j_hi - 1 - j_lo) / span;
+            }
+            if (spc_r > 0 && spc_d > 0 && cp * spc_r > 4.0 * (0.5 - C.half_p) &&
+                st * spc_d > 4.0 * (0.5 - C.half_t) && q0 >= C.walk_min_q) {
+                WalkTask T;
+                T.cos_half = cos_half;
+                T.sin_half = sin_half;
+                T.cp = cp;
+                T.st = st;
+                T.rc = uniform_f64(F.rd[ob + (lb >> 1)].x);
+                T.dc = uniform_f64(F.rd[ob + (lb >> 1)].y);
+                T.j_lo = j_lo;
+                T.j_hi = j_hi;
+                T.q0 = q0;
+                DeviceWalkMem mem{F.rd + ob, F.pre + (ob + b), F.w + ob, F.z + ob, F.dw + ob,
+                                  hist, lds_stats, 0u, 0u, 0.0};
+                if (active) {
+                    const WalkLane L{r1, d1, zh1, w1, dw1};
+                    walk_lane(C, T, L, mem);
+                }
+                cnt.visit += (uint64_t)mem.cover;
+                cnt.runs += mem.runs;
+                cnt.probes += mem.probes;
+                ++n_walk;
+                if (j_hi > j_lo) ++n_steps;
+                return;
+            }
+        }
         // Pixel j of delta2 is wave-uniform: its record comes through the scalar cache into
         // SGPRs and the loop touches LDS only to flush runs.  Two pixels per iteration, the
         // next two already loading; the store is padded, so reading up to three records
@@ -494,9 +590,9 @@ xi_kernel(XiConst C, ForestView F, PairView P, GroupView G, XiOut O) {
     if (tid < XI_NSTATS) s_stats[tid] = 0;
     if (tid == 0) s_const = C;  // published by the first __syncthreads of the group loop
 
-    LaneCounters cnt = {0, 0, 0, 0};
+    LaneCounters cnt = {0, 0, 0, 0, 0, 0};
     uint64_t accepted = 0;  // LDS_HIST: pairs moved from the histogram to HBM
-    uint64_t n_steps = 0, n_tasks = 0, n_empty_tasks = 0;  // wave-uniform
+    uint64_t n_steps = 0, n_tasks = 0, n_empty_tasks = 0, n_walk = 0;  // wave-uniform
     int current_row = -1;
 
     for (;;) {
@@ -550,11 +646,11 @@ xi_kernel(XiConst C, ForestView F, PairView P, GroupView G, XiOut O) {
             const uint64_t steps_before = n_steps;
             if (LDS_HIST) {
                 xi_task<MODE, XCORR, GENERAL, true>(C, s_const, F, P, p0 + q, chunk, lane, hist,
-                                                    tile, tile_x, cnt, lds_stats, n_steps);
+                                                    tile, tile_x, cnt, lds_stats, n_steps, n_walk);
             } else {
                 xi_task<MODE, XCORR, GENERAL, false>(C, s_const, F, P, p0 + q, chunk, lane,
                                                      O.bins + (int64_t)row * C.nb, tile, tile_x,
-                                                     cnt, lds_stats, n_steps);
+                                                     cnt, lds_stats, n_steps, n_walk);
             }
             ++n_tasks;
             if (n_steps == steps_before) ++n_empty_tasks;
@@ -569,6 +665,8 @@ xi_kernel(XiConst C, ForestView F, PairView P, GroupView G, XiOut O) {
     const uint64_t v_edge = wave_sum_u64(cnt.edge);
     const uint64_t v_over = wave_sum_u64(cnt.overflow);
     const uint64_t v_accept = wave_sum_u64(accepted);
+    const uint64_t v_runs = wave_sum_u64(cnt.runs);
+    const uint64_t v_probes = wave_sum_u64(cnt.probes);
     if (lane == 0) {
         atomicAdd(&s_stats[XI_STAT_VISITED], (unsigned long long)v_visit);
         atomicAdd(&s_stats[XI_STAT_EDGE], (unsigned long long)v_edge);
@@ -577,6 +675,9 @@ xi_kernel(XiConst C, ForestView F, PairView P, GroupView G, XiOut O) {
         atomicAdd(&s_stats[XI_STAT_ACCEPTED], (unsigned long long)v_accept);
         atomicAdd(&s_stats[XI_STAT_TASKS], (unsigned long long)n_tasks);
         atomicAdd(&s_stats[XI_STAT_EMPTY_TASKS], (unsigned long long)n_empty_tasks);
+        atomicAdd(&s_stats[XI_STAT_RUNS], (unsigned long long)v_runs);
+        atomicAdd(&s_stats[XI_STAT_PROBES], (unsigned long long)v_probes);
+        atomicAdd(&s_stats[XI_STAT_WALK_TASKS], (unsigned long long)n_walk);
     }
     __syncthreads();
     if (tid < XI_NSTATS) atomicAdd(O.stats + tid, s_stats[tid]);

@@ -155,6 +155,12 @@ class Engine:
         (we, xi, rp, rt, z), nb = self.xi_download(params)
         return XiResult(we, xi, rp, rt, z, nb, report.as_dict())
 
+    def set_xi_schedule(self, schedule):
+        """Schedule of the specialised pair kernel: _native.XI_SCHEDULE_AUTO, _PAIRS or
+        _RUNS (picca_xi_set_schedule)."""
+        check(self._lib.picca_xi_set_schedule(self._ctx, int(schedule)),
+              "picca_xi_set_schedule")
+
     def xi_execute(self, params):
         report = _native.XiReport()
         check(self._lib.picca_xi_execute(self._ctx, ctypes.byref(params), ctypes.byref(report)),
