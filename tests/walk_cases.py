@@ -86,10 +86,14 @@ def _store(forests):
 
 
 def _all_pairs(n, ang):
+    """Every ordered pair of n forests, spread over three output rows (contiguous, as the
+    engine requires)."""
     a, b = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
     keep = a != b
     pa, pb = a[keep], b[keep]
-    return pa, pb, np.full(len(pa), ang), (pa % 3).astype(np.int32)
+    row = (pa % 3).astype(np.int32)
+    order = np.argsort(row, kind="stable")
+    return pa[order], pb[order], np.full(len(pa), ang), row[order]
 
 
 def lattice_case():
