@@ -138,6 +138,21 @@ int picca_xi_set_pairs(picca_ctx *ctx, int64_t num_pairs, const int32_t *pair_a,
 int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_report *report);
 
 /*
+ * picca_xi_execute for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc); no counterpart in picca.  The
+ * sums go to the caller's device buffer out_bins: num_rows * np*nt records of 48 bytes
+ * (sum w1 w2, sum w1 d1 w2 d2, sum r_par w1 w2, sum r_trans w1 w2, sum z w1 w2 as FP64, the
+ * pair count as uint64), out_capacity records long; a smaller buffer is rejected
+ * (PICCA_E_VALUE).  The work-queue head and the counters live in a fixed device array of
+ * 32 words that the library holds per GPU for the life of the process, whatever the input
+ * size; it is cleared at every call, and calls that overlap on one GPU race on it.  The
+ * forests and pairs must already be resident (picca_set_forests, picca_xi_set_pairs).
+ * Calls no hipMalloc or hipFree; the result is not kept for picca_xi_download.
+ */
+int picca_xi_execute_noalloc(picca_ctx *ctx, const picca_cf_params *params, void *out_bins,
+                             int64_t out_capacity, picca_xi_report *report);
+
+/*
  * Schedule of the pair kernel's specialised path (standard binning, auto-correlation,
  * r_par_min == 0, no cuts).  RUNS walks each pixel of delta1 through runs of pixels of
  * delta2 that share one bin, summed from per-forest prefix sums, wherever delta2's pixels

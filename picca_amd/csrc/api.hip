@@ -89,6 +89,13 @@ int picca_create(int32_t device, picca_ctx **out) {
     }
     ctx->num_cus = prop.multiProcessorCount;
     ctx->lds_limit = (int64_t)prop.maxSharedMemoryPerMultiProcessor;
+    // this GPU's copy of the no-allocation scratch array (device `device` is current)
+    err = picca::xi_scratch_address(&ctx->scratch);
+    if (err != hipSuccess) {
+        picca_destroy(ctx);
+        return fail(PICCA_E_HIP, std::string("picca_create: scratch address: ") +
+                                     hipGetErrorString(err));
+    }
     *out = ctx;
     return PICCA_OK;
 }
@@ -314,16 +321,15 @@ int picca_xi_set_pairs(picca_ctx *ctx, int64_t num_pairs, const int32_t *pair_a,
     return PICCA_OK;
 }
 
-int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_report *report) {
-    if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
-    picca::XiConst C;
-    int mode = 0;
-    int status = build_xi_const(params, C, mode);
-    if (status != PICCA_OK) return status;
-    {
-        const char *err = picca::widen_xi_margins(C, ctx->max_abs_r, ctx->max_abs_d);
-        if (err) return fail(PICCA_E_VALUE, err);
-    }
+namespace {
+
+// picca_xi_execute's work with the output rows, the work-queue head and the counters in the
+// given device buffers (out: num_rows * np*nt records; queue: 1 word; stats: XI_NSTATS
+// words), all cleared here by kernels.  Allocates nothing.
+int xi_execute_into(picca_ctx *ctx, const picca::XiConst &C0, int mode,
+                    const picca_cf_params *params, picca::XiBin *out, unsigned long long *queue,
+                    unsigned long long *stats_dev, picca_xi_report *report) {
+    picca::XiConst C = C0;
     // the specialised kernel: standard binning, auto-correlation, r_par_min == 0, no cuts
     const bool general = !picca::xi_specialised(params, mode);
     picca::apply_xi_schedule(C, ctx->xi_schedule);
@@ -357,16 +363,12 @@ int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_rep
         C.tile_len = 64;
         lds = lds_bytes(false, 64);
     }
-    PICCA_HIP_CHECK(hipSetDevice(ctx->device));
     const int64_t nrows = ctx->num_rows;
     const int64_t nout = nrows * C.nb;
-    PICCA_HIP_CHECK(ctx->o_bins.reserve((size_t)nout));
-    PICCA_HIP_CHECK(ctx->queue.reserve(1));
-    PICCA_HIP_CHECK(ctx->stats.reserve(picca::XI_NSTATS));
-    PICCA_HIP_CHECK(picca::launch_zero_u64(reinterpret_cast<unsigned long long *>(ctx->o_bins.ptr),
+    PICCA_HIP_CHECK(picca::launch_zero_u64(reinterpret_cast<unsigned long long *>(out),
                                            nout * (int64_t)(sizeof(picca::XiBin) / 8), ctx->stream));
-    PICCA_HIP_CHECK(picca::launch_zero_u64(ctx->queue.ptr, 1, ctx->stream));
-    PICCA_HIP_CHECK(picca::launch_zero_u64(ctx->stats.ptr, picca::XI_NSTATS, ctx->stream));
+    PICCA_HIP_CHECK(picca::launch_zero_u64(queue, 1, ctx->stream));
+    PICCA_HIP_CHECK(picca::launch_zero_u64(stats_dev, picca::XI_NSTATS, ctx->stream));
 
     picca::ForestView F{ctx->pix.ptr, ctx->z.ptr, ctx->r.ptr, ctx->d.ptr, ctx->w.ptr, ctx->dw.ptr,
                         ctx->ll.ptr, ctx->off.ptr, ctx->zq.ptr, ctx->order.ptr, ctx->mono.ptr,
@@ -374,7 +376,7 @@ int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_rep
     picca::PairView P{ctx->pa.ptr, ctx->pb.ptr, ctx->pang.ptr, ctx->pcs.ptr, ctx->pflags.ptr};
     const int64_t ngroups = (int64_t)ctx->h_group_row.size();
     picca::GroupView G{ctx->gbegin.ptr, ctx->grow.ptr, ngroups};
-    picca::XiOut O{ctx->o_bins.ptr, ctx->queue.ptr, ctx->stats.ptr};
+    picca::XiOut O{out, queue, stats_dev};
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->num_cus, ngroups));
     PICCA_HIP_CHECK(hipEventRecord(ctx->ev_start, ctx->stream));
     if (ngroups > 0) {
@@ -383,14 +385,11 @@ int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_rep
     }
     PICCA_HIP_CHECK(hipEventRecord(ctx->ev_stop, ctx->stream));
     unsigned long long stats[picca::XI_NSTATS];
-    PICCA_HIP_CHECK(hipMemcpyAsync(stats, ctx->stats.ptr, sizeof(stats), hipMemcpyDeviceToHost,
+    PICCA_HIP_CHECK(hipMemcpyAsync(stats, stats_dev, sizeof(stats), hipMemcpyDeviceToHost,
                                    ctx->stream));
     PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     float ms = 0.f;
     PICCA_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
-    ctx->out_nb = C.nb;
-    ctx->out_rows = (int32_t)nrows;
-    ctx->have_result = true;
     if (report) {
         report->accepted_pairs = (int64_t)stats[picca::XI_STAT_ACCEPTED];
         report->edge_pairs = (int64_t)stats[picca::XI_STAT_EDGE];
@@ -406,6 +405,58 @@ int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_rep
         report->launches = ngroups > 0 ? 1 : 0;
     }
     return PICCA_OK;
+}
+
+}  // namespace
+
+int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_report *report) {
+    if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
+    picca::XiConst C;
+    int mode = 0;
+    int status = build_xi_const(params, C, mode);
+    if (status != PICCA_OK) return status;
+    {
+        const char *err = picca::widen_xi_margins(C, ctx->max_abs_r, ctx->max_abs_d);
+        if (err) return fail(PICCA_E_VALUE, err);
+    }
+    PICCA_HIP_CHECK(hipSetDevice(ctx->device));
+    ctx->have_result = false;
+    PICCA_HIP_CHECK(ctx->o_bins.reserve((size_t)ctx->num_rows * (size_t)C.nb));
+    PICCA_HIP_CHECK(ctx->queue.reserve(1));
+    PICCA_HIP_CHECK(ctx->stats.reserve(picca::XI_NSTATS));
+    status = xi_execute_into(ctx, C, mode, params, ctx->o_bins.ptr, ctx->queue.ptr,
+                             ctx->stats.ptr, report);
+    if (status == PICCA_OK) {
+        ctx->out_nb = C.nb;
+        ctx->out_rows = ctx->num_rows;
+        ctx->have_result = true;
+    }
+    return status;
+}
+
+int picca_xi_execute_noalloc(picca_ctx *ctx, const picca_cf_params *params, void *out_bins,
+                             int64_t out_capacity, picca_xi_report *report) {
+    if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
+    picca::XiConst C;
+    int mode = 0;
+    int status = build_xi_const(params, C, mode);
+    if (status != PICCA_OK) return status;
+    {
+        const char *err = picca::widen_xi_margins(C, ctx->max_abs_r, ctx->max_abs_d);
+        if (err) return fail(PICCA_E_VALUE, err);
+    }
+    const int64_t need = (int64_t)ctx->num_rows * C.nb;
+    if (need > 0 && !out_bins) return fail(PICCA_E_VALUE, "out_bins is NULL");
+    if (out_capacity < need)
+        return fail(PICCA_E_VALUE, "out_bins holds " + std::to_string(out_capacity) +
+                                       " records, num_rows * np * nt = " +
+                                       std::to_string(need) + " are needed");
+    if (!ctx->scratch)
+        return fail(PICCA_E_RUNTIME, "the device scratch array of this GPU is not available");
+    PICCA_HIP_CHECK(hipSetDevice(ctx->device));
+    // scratch: word 0 the work-queue head, words 1.. the counters
+    return xi_execute_into(ctx, C, mode, params, (picca::XiBin *)out_bins, ctx->scratch,
+                           ctx->scratch + 1, report);
 }
 
 int picca_xi_set_schedule(picca_ctx *ctx, int32_t schedule) {

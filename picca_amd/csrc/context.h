@@ -104,6 +104,7 @@ struct picca_ctx {
     hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_mid = nullptr;
     int num_cus = 0;
     int64_t lds_limit = 0;
+    unsigned long long *scratch = nullptr;   // this GPU's no-allocation scratch array
 
     // forest store
     int64_t num_forests = 0;

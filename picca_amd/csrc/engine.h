@@ -115,6 +115,10 @@ struct XiBin {
     unsigned long long nb;
 };
 
+// 64-bit words of the no-allocation variant's device array: queue head + counters
+constexpr int XI_SCRATCH_WORDS = 32;
+static_assert(1 + XI_NSTATS <= XI_SCRATCH_WORDS, "xi scratch too small");
+
 struct XiOut {
     XiBin *bins;                 // [num_rows * nb]
     unsigned long long *queue;   // work-queue head (zeroed before each launch)
@@ -493,6 +497,8 @@ hipError_t launch_xi(int mode, bool xcorr, bool general, bool lds_hist, const Xi
                      const ForestView &F, const PairView &P, const GroupView &G,
                      const XiOut &O, int grid, size_t lds, hipStream_t stream);
 hipError_t launch_zero_u64(unsigned long long *ptr, int64_t n, hipStream_t stream);
+// device address of the current GPU's scratch array (XI_SCRATCH_WORDS words)
+hipError_t xi_scratch_address(unsigned long long **ptr);
 hipError_t launch_copy16(const void *src, void *dst, int64_t n16, hipStream_t stream);
 
 }  // namespace picca

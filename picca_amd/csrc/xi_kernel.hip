@@ -683,6 +683,18 @@ xi_kernel(XiConst C, ForestView F, PairView P, GroupView G, XiOut O) {
     if (tid < XI_NSTATS) atomicAdd(O.stats + tid, s_stats[tid]);
 }
 
+// Work-queue head and counters of picca_xi_execute_noalloc: one copy per GPU, held until
+// the process ends.  Host code reaches it only through the address hipGetSymbolAddress
+// returns (xi_scratch_address), never through this name.
+static __device__ __align__(256) unsigned long long scratch[XI_SCRATCH_WORDS];
+
+hipError_t xi_scratch_address(unsigned long long **ptr) {
+    void *p = nullptr;
+    hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    *ptr = err == hipSuccess ? (unsigned long long *)p : nullptr;
+    return err;
+}
+
 __global__ void zero_u64_kernel(unsigned long long *__restrict__ a, int64_t n) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
