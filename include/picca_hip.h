@@ -249,6 +249,36 @@ int picca_cont_fit(picca_ctx *ctx, const picca_cont_params *params, const double
                    int32_t *status, int32_t *evaluations, double *kernel_ms);
 
 /*
+ * The mean-continuum update of the DR16 iteration (Dr16ExpectedFlux.compute_expected_flux,
+ * py/picca/delta_extraction/expected_fluxes/dr16_expected_flux.py:380-464).
+ *
+ * picca_cont_set_rest_frame sets, for the resident forests, each pixel's rest-frame
+ * log lambda, its transmission correction and its bin of the rest-frame grid (num_bins
+ * bins, find_bins_log / find_bins_lin of py/picca/delta_extraction/utils.py:139-186).
+ *
+ * picca_cont_stack stacks the last fit: over the pixels of the forests with status 0,
+ * sum_ratio[bin] = sum F / (C + 1e-16) w and sum_weight[bin] = sum w, with C the fitted
+ * continuum and w the forest weight 1 / (eta var_pipe + var_lss + fudge / var_pipe),
+ * var_pipe = 1 / ivar / C^2, 0 where ivar <= 0 (ExpectedFlux._compute_mean_cont,
+ * py/picca/delta_extraction/expected_flux.py:256-308, with compute_forest_weights,
+ * dr16_expected_flux.py:466-488).  The caller normalises the stack and multiplies it into
+ * the previous mean continuum (a few hundred bins).
+ *
+ * picca_cont_set_mean_cont sets every pixel's mean continuum to the linear interpolation
+ * of the nodes (x, y) (n >= 2, x increasing; the end intervals extend beyond them, as
+ * scipy's interp1d with fill_value="extrapolate") at its rest-frame log lambda, times its
+ * transmission correction; the next picca_cont_fit uses it.
+ */
+#define PICCA_CONT_MAX_BINS 8192
+int picca_cont_set_rest_frame(picca_ctx *ctx, const double *log_lambda_rest,
+                              const double *transmission, const int32_t *bins,
+                              int32_t num_bins);
+int picca_cont_stack(picca_ctx *ctx, double *sum_ratio, double *sum_weight,
+                     double *kernel_ms);
+int picca_cont_set_mean_cont(picca_ctx *ctx, int32_t n, const double *x, const double *y,
+                             double *kernel_ms);
+
+/*
  * Seed network of the continuum fit (no counterpart in picca: it predicts the start
  * (a, b) of the fit above from the binned spectrum).  A two-layer perceptron
  * y = relu(x W1 + b1) W2 + b2 with n_in inputs (multiple of 32, <= 256), n_hidden units

@@ -26,6 +26,15 @@ FP64 numpy restatement of picca's continuum fit for one forest:
   ``Dr16ExpectedFlux.compute_forest_weights``
   (py/picca/delta_extraction/expected_fluxes/dr16_expected_flux.py:466-488) and the
   bins of ``find_bins_log`` / ``find_bins_lin`` (py/picca/delta_extraction/utils.py:139-186).
+* ``rest_frame_grid_log`` -- ``Forest.log_lambda_rest_frame_grid`` of the log wavelength
+  solution (py/picca/delta_extraction/astronomical_objects/forest.py:729-732).
+* ``iterate`` -- the loop of ``Dr16ExpectedFlux.compute_expected_flux``
+  (py/picca/delta_extraction/expected_fluxes/dr16_expected_flux.py:380-464): fit every
+  forest with the current mean continuum, then, except after the last iteration, re-stack
+  the mean continuum and replace the interpolator (scipy ``interp1d`` with
+  ``fill_value="extrapolate"``, as ``_compute_mean_cont`` does).  eta, var_lss and fudge
+  stay as given: the reference skips ``compute_var_stats`` with ``use constant weight`` or
+  ``use ivar as weight``, and the Dr16Fixed* classes hold them.
 """
 import numpy as np
 from scipy.optimize import minimize
@@ -184,3 +193,40 @@ def stack_mean_cont(specs, conts, grid, bins_of, previous):
     mean_cont[w] /= mean_weight[w]
     mean_cont /= mean_cont[w].mean()
     return grid[w], previous[w] * mean_cont[w], mean_weight[w]
+
+
+def rest_frame_grid_log(lambda_min, lambda_max, step):
+    return np.arange(np.log10(lambda_min), np.log10(lambda_max) + step / 2, step)
+
+
+def mean_cont_interpolator(x, y):
+    from scipy.interpolate import interp1d
+    return interp1d(x, y, fill_value="extrapolate")
+
+
+def iterate(specs, grid, num_iterations, order=1, use_constant_weight=False,
+            bins_of=find_bins_log):
+    """Returns (fits of the last iteration [(a, b, chi2, ndata, valid)], kept grid points,
+    mean continuum there, history of the mean continuum after each re-stack).  The specs'
+    ``mean_cont`` entries are overwritten with the interpolated mean continuum (times
+    ``transmission`` when a spec holds one)."""
+    x_keep, y_keep = grid, np.ones_like(grid)
+    history = []
+    fits = []
+    for iteration in range(num_iterations):
+        get_mean_cont = mean_cont_interpolator(x_keep, y_keep)
+        for spec in specs:
+            ll_rest = spec["log_lambda"] - np.log10(1 + spec["z_qso"])
+            spec["mean_cont"] = get_mean_cont(ll_rest) * spec.get("transmission", 1.0)
+        fits = [fit(spec, order=order, use_constant_weight=use_constant_weight)
+                for spec in specs]
+        if iteration < num_iterations - 1:
+            conts = []
+            for spec, (a, b, _, _, valid) in zip(specs, fits):
+                conts.append(continuum_model(a, b, spec["log_lambda"], spec["log_lambda_min"],
+                                             spec["log_lambda_max"], spec["mean_cont"])
+                             if valid else None)
+            x_keep, y_keep, _ = stack_mean_cont(specs, conts, grid, bins_of,
+                                                get_mean_cont(grid))
+            history.append((x_keep, y_keep))
+    return fits, x_keep, y_keep, history

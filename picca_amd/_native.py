@@ -145,6 +145,9 @@ _PROTOTYPES = {
     "picca_xcf_execute": [_P, _P, _P],
     "picca_cont_set_forests": [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "picca_cont_fit": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "picca_cont_set_rest_frame": [_P, _P, _P, _P, _I32],
+    "picca_cont_stack": [_P, _P, _P, _P],
+    "picca_cont_set_mean_cont": [_P, _I32, _P, _P, _P],
     "picca_mlp_init": [_P, _I32, _I32, _P, _P, _P, _P],
     "picca_mlp_get": [_P, _P, _P, _P, _P],
     "picca_mlp_forward": [_P, _I64, _P, _P, _P],

@@ -11,6 +11,16 @@ The fit itself (``picca_cont_fit``) minimises the reference's chi2 by damped New
 iterations on the GPU, one wave per forest; it starts where Migrad starts (a0 = weighted
 mean flux, b0 = 0) unless start values are given (for example the learned seeds of
 ``picca_amd.seed_mlp``).
+
+``compute_expected_flux`` runs the iteration of ``Dr16ExpectedFlux.compute_expected_flux``
+(py/picca/delta_extraction/expected_fluxes/dr16_expected_flux.py:380-464) on the GPU:
+fit every forest with the current mean continuum, then (except after the last iteration)
+stack F / C in rest-frame bins with the forest weights of ``compute_forest_weights``
+(:466-488), normalise the stack, multiply it into the previous mean continuum
+(py/picca/delta_extraction/expected_flux.py:256-308) and interpolate the result back to
+every pixel.  eta, var_lss and fudge stay as given, as in the reference with ``use
+constant weight`` or ``use ivar as weight`` (which skip ``compute_var_stats``) and in the
+Dr16Fixed* classes.
 """
 import numpy as np
 
@@ -104,3 +114,73 @@ def compute_continuum(forest, get_mean_cont, get_eta, get_var_lss, get_fudge,
     """picca's compute_continuum for one forest (a batch of one on the GPU)."""
     return compute_continua([forest], get_mean_cont, get_eta, get_var_lss, get_fudge,
                             use_constant_weight, order, rest_frame_grid)[0]
+
+
+def rest_frame_grid_log(lambda_min, lambda_max, step):
+    """Forest.log_lambda_rest_frame_grid of the log wavelength solution
+    (py/picca/delta_extraction/astronomical_objects/forest.py:729-732)."""
+    return np.arange(np.log10(lambda_min), np.log10(lambda_max) + step / 2, step)
+
+
+def find_bins_log(log_lambda_rest, grid):
+    """find_bins_log (py/picca/delta_extraction/utils.py:162-186)."""
+    step = grid[1] - grid[0]
+    return ((log_lambda_rest - grid[0]) / step + 0.5).astype(np.int64)
+
+
+def compute_expected_flux(inputs, grid, num_iterations, order=1, use_constant_weight=False,
+                          transmission=None, bins_of=find_bins_log, max_iter=100, tol=1e-12,
+                          tol_abs=0.0, seeds=None):
+    """The DR16 iteration over the flat forest inputs of ``synth_spectra.forest_inputs``
+    (per pixel: log_lambda, flux, ivar, x, eta, var_lss, fudge; per forest: z_qso).
+
+    Returns a dict: ``fit`` (the last iteration's fit, as ``fit_arrays``), ``x`` / ``y``
+    (the mean continuum at the kept rest-frame grid points), ``history`` (the mean
+    continuum after each re-stack) and ``kernel_ms`` (device time of every fit, stack and
+    interpolation).  ``seeds(iteration, inputs)`` may return start values (a0, b0) for an
+    iteration's fit."""
+    offsets = np.asarray(inputs["offsets"], dtype=np.int64)
+    n = len(offsets) - 1
+    seg = np.repeat(np.arange(n), np.diff(offsets))
+    ll_rest = np.asarray(inputs["log_lambda"], dtype=np.float64) - np.log10(
+        1 + np.asarray(inputs["z_qso"], dtype=np.float64))[seg]
+    bins = np.asarray(bins_of(ll_rest, grid), dtype=np.int64)
+    if bins.size and (bins.min() < 0 or bins.max() >= grid.size):
+        raise ValueError("pixels outside the rest-frame grid")
+    trans = np.ones_like(ll_rest) if transmission is None else np.asarray(transmission, float)
+    orders = np.broadcast_to(np.asarray(order, dtype=np.int32), (n,)).copy()
+    eng = _engine.get_engine(device)
+    eng.cont_set_forests(offsets, inputs["flux"], inputs["ivar"], np.ones_like(ll_rest),
+                         inputs["x"], inputs["eta"], inputs["var_lss"], inputs["fudge"], orders)
+    eng.cont_set_rest_frame(ll_rest, trans, bins, grid.size)
+    x_keep, y_keep = np.asarray(grid, dtype=np.float64), np.ones(grid.size)
+    history, kernel_ms = [], 0.0
+    res = None
+    for iteration in range(num_iterations):
+        kernel_ms += eng.cont_set_mean_cont(x_keep, y_keep)
+        if seeds is not None and seeds(iteration, inputs) is not None:
+            a0, b0 = seeds(iteration, inputs)
+        else:
+            a0, b0 = start_values(offsets, inputs["flux"], inputs["ivar"])
+        res = eng.cont_fit(a0, b0, max_iter=max_iter, tol=tol,
+                           use_constant_weight=use_constant_weight, tol_abs=tol_abs)
+        kernel_ms += res["kernel_ms"]
+        if iteration < num_iterations - 1:
+            total, weight, ms = eng.cont_stack(grid.size)
+            kernel_ms += ms
+            keep = weight > 0
+            stack = np.zeros(grid.size)
+            stack[keep] = total[keep] / weight[keep]
+            stack /= stack[keep].mean()
+            previous = _interp1d(x_keep, y_keep, grid)
+            x_keep, y_keep = grid[keep], previous[keep] * stack[keep]
+            history.append((x_keep, y_keep))
+    return {"fit": res, "x": x_keep, "y": y_keep, "history": history, "kernel_ms": kernel_ms}
+
+
+def _interp1d(x, y, v):
+    """scipy's interp1d(x, y, fill_value="extrapolate") at v (its _call_linear)."""
+    idx = np.clip(np.searchsorted(x, v), 1, len(x) - 1)
+    lo, hi = idx - 1, idx
+    slope = (y[hi] - y[lo]) / (x[hi] - x[lo])
+    return slope * (v - x[lo]) + y[lo]

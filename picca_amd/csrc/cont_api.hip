@@ -54,6 +54,8 @@ int picca_cont_set_forests(picca_ctx *ctx, int64_t num_forests, const int64_t *o
     PICCA_HIP_CHECK(hipStreamSynchronize(st));
     ctx->cont_forests = num_forests;
     ctx->cont_pixels = npix;
+    ctx->cont_nbins = 0;
+    ctx->cont_fitted = false;
     return PICCA_OK;
 }
 
@@ -96,6 +98,99 @@ int picca_cont_fit(picca_ctx *ctx, const picca_cont_params *params, const double
     float ms = 0.f;
     PICCA_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
     if (kernel_ms) *kernel_ms = (double)ms;
+    ctx->cont_fitted = true;
+    return PICCA_OK;
+}
+
+int picca_cont_set_rest_frame(picca_ctx *ctx, const double *log_lambda_rest,
+                              const double *transmission, const int32_t *bins,
+                              int32_t num_bins) {
+    if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
+    const int64_t npix = ctx->cont_pixels;
+    if (num_bins < 2 || num_bins > PICCA_CONT_MAX_BINS)
+        return fail(PICCA_E_VALUE, "num_bins must be in [2, " +
+                                       std::to_string(PICCA_CONT_MAX_BINS) + "]");
+    if (npix > 0 && (!log_lambda_rest || !transmission || !bins))
+        return fail(PICCA_E_VALUE, "rest-frame arrays must not be NULL");
+    for (int64_t k = 0; k < npix; ++k) {
+        if (bins[k] < 0 || bins[k] >= num_bins)
+            return fail(PICCA_E_VALUE, "a pixel's rest-frame bin is outside [0, num_bins)");
+        if (!std::isfinite(log_lambda_rest[k]) || !std::isfinite(transmission[k]))
+            return fail(PICCA_E_VALUE, "rest-frame inputs must be finite");
+    }
+    PICCA_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    PICCA_HIP_CHECK(ctx->c_llr.upload(log_lambda_rest, (size_t)npix, st));
+    PICCA_HIP_CHECK(ctx->c_trans.upload(transmission, (size_t)npix, st));
+    PICCA_HIP_CHECK(ctx->c_bin.upload(bins, (size_t)npix, st));
+    PICCA_HIP_CHECK(ctx->c_stack.reserve(2 * (size_t)num_bins));
+    PICCA_HIP_CHECK(ctx->c_xs.reserve((size_t)num_bins));
+    PICCA_HIP_CHECK(ctx->c_ys.reserve((size_t)num_bins));
+    PICCA_HIP_CHECK(hipStreamSynchronize(st));
+    ctx->cont_nbins = num_bins;
+    return PICCA_OK;
+}
+
+int picca_cont_stack(picca_ctx *ctx, double *sum_ratio, double *sum_weight,
+                     double *kernel_ms) {
+    if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
+    if (ctx->cont_nbins == 0)
+        return fail(PICCA_E_RUNTIME, "no rest frame set (picca_cont_set_rest_frame)");
+    if (!ctx->cont_fitted) return fail(PICCA_E_RUNTIME, "no fit of the resident forests");
+    if (!sum_ratio || !sum_weight) return fail(PICCA_E_VALUE, "outputs must not be NULL");
+    PICCA_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int32_t nbins = ctx->cont_nbins;
+    double *d_ratio = ctx->c_stack.ptr, *d_weight = ctx->c_stack.ptr + nbins;
+    PICCA_HIP_CHECK(picca::launch_zero_u64(reinterpret_cast<unsigned long long *>(d_ratio),
+                                           2 * (int64_t)nbins, st));
+    picca::ContIn I{ctx->c_off.ptr,  ctx->c_flux.ptr, ctx->c_ivar.ptr,  ctx->c_mean.ptr,
+                    ctx->c_x.ptr,    ctx->c_eta.ptr,  ctx->c_lss.ptr,   ctx->c_fudge.ptr,
+                    ctx->c_order.ptr, ctx->c_a0.ptr,  ctx->c_b0.ptr};
+    PICCA_HIP_CHECK(hipEventRecord(ctx->ev_start, st));
+    PICCA_HIP_CHECK(picca::launch_cont_stack(I, ctx->c_a.ptr, ctx->c_b.ptr, ctx->c_status.ptr,
+                                             ctx->c_bin.ptr, nbins, d_ratio, d_weight,
+                                             ctx->cont_forests, ctx->num_cus, st));
+    PICCA_HIP_CHECK(hipEventRecord(ctx->ev_stop, st));
+    PICCA_HIP_CHECK(hipMemcpyAsync(sum_ratio, d_ratio, sizeof(double) * nbins,
+                                   hipMemcpyDeviceToHost, st));
+    PICCA_HIP_CHECK(hipMemcpyAsync(sum_weight, d_weight, sizeof(double) * nbins,
+                                   hipMemcpyDeviceToHost, st));
+    PICCA_HIP_CHECK(hipStreamSynchronize(st));
+    float ms = 0.f;
+    PICCA_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    if (kernel_ms) *kernel_ms = (double)ms;
+    return PICCA_OK;
+}
+
+int picca_cont_set_mean_cont(picca_ctx *ctx, int32_t n, const double *x, const double *y,
+                             double *kernel_ms) {
+    if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
+    if (ctx->cont_nbins == 0)
+        return fail(PICCA_E_RUNTIME, "no rest frame set (picca_cont_set_rest_frame)");
+    if (n < 2 || n > ctx->cont_nbins)
+        return fail(PICCA_E_VALUE, "the mean continuum needs 2 to num_bins nodes");
+    if (!x || !y) return fail(PICCA_E_VALUE, "nodes must not be NULL");
+    for (int32_t k = 0; k < n; ++k) {
+        if (!std::isfinite(x[k]) || !std::isfinite(y[k]))
+            return fail(PICCA_E_VALUE, "nodes must be finite");
+        if (k > 0 && !(x[k] > x[k - 1]))
+            return fail(PICCA_E_VALUE, "node positions must increase strictly");
+    }
+    PICCA_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    PICCA_HIP_CHECK(ctx->c_xs.upload(x, (size_t)n, st));
+    PICCA_HIP_CHECK(ctx->c_ys.upload(y, (size_t)n, st));
+    PICCA_HIP_CHECK(hipEventRecord(ctx->ev_start, st));
+    PICCA_HIP_CHECK(picca::launch_cont_interp(ctx->c_llr.ptr, ctx->c_trans.ptr, ctx->c_xs.ptr,
+                                              ctx->c_ys.ptr, n, ctx->c_mean.ptr,
+                                              ctx->cont_pixels, ctx->num_cus, st));
+    PICCA_HIP_CHECK(hipEventRecord(ctx->ev_stop, st));
+    PICCA_HIP_CHECK(hipStreamSynchronize(st));
+    float ms = 0.f;
+    PICCA_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    if (kernel_ms) *kernel_ms = (double)ms;
+    ctx->cont_fitted = false;   // the resident fit belongs to the old mean continuum
     return PICCA_OK;
 }
 

@@ -190,4 +190,126 @@ hipError_t launch_cont_fit(const ContConst &K, const ContIn &I, const ContOut &O
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------------------
+// Mean-continuum update of the DR16 iteration
+// (Dr16ExpectedFlux.compute_expected_flux, py/picca/delta_extraction/expected_fluxes/
+// dr16_expected_flux.py:380-464).
+//
+// cont_stack_kernel: ExpectedFlux._compute_mean_cont (py/picca/delta_extraction/
+// expected_flux.py:256-308) with the forest weights of Dr16ExpectedFlux.compute_forest_weights
+// (dr16_expected_flux.py:466-488, var_lss_mod = 1): over the pixels of the forests whose
+// last fit is valid, sum F / (C + 1e-16) * w and w in the pixel's rest-frame bin, with
+// w = 1 / (eta var_pipe + var_lss + fudge / var_pipe), var_pipe = 1 / ivar / C^2 and
+// w = 0 where ivar <= 0.  One wave per forest; the bins live in LDS per workgroup and are
+// added to HBM once per workgroup.
+__global__ void __launch_bounds__(256)
+cont_stack_kernel(ContIn I, const double *__restrict__ a_fit, const double *__restrict__ b_fit,
+                  const int32_t *__restrict__ status, const int32_t *__restrict__ bins,
+                  int32_t nbins, double *__restrict__ sum_ratio,
+                  double *__restrict__ sum_weight, int64_t n_forests) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *h_ratio = reinterpret_cast<double *>(smem);
+    double *h_weight = h_ratio + nbins;
+    for (int k = threadIdx.x; k < nbins; k += blockDim.x) {
+        h_ratio[k] = 0.0;
+        h_weight[k] = 0.0;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t f = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); f < n_forests;
+         f += waves) {
+        if (status[f] != 0) continue;
+        const double a = a_fit[f], b = b_fit[f];
+        const int64_t o = I.off[f];
+        const int n = (int)(I.off[f + 1] - o);
+        for (int k = lane; k < n; k += 64) {
+            const double iv = I.ivar[o + k];
+            if (!(iv > 0)) continue;
+            const double C = (b * I.x[o + k] + a) * I.mean_cont[o + k];
+            const double var_pipe = 1.0 / iv / (C * C);
+            const double w = 1.0 / (I.eta[o + k] * var_pipe + I.var_lss[o + k] +
+                                    I.fudge[o + k] / var_pipe);
+            const double ratio = I.flux[o + k] / (C + 1e-16);
+            const int bin = bins[o + k];
+            atomicAdd(&h_ratio[bin], ratio * w);
+            atomicAdd(&h_weight[bin], w);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nbins; k += blockDim.x) {
+        if (h_weight[k] != 0.0) {
+            unsafeAtomicAdd(&sum_ratio[k], h_ratio[k]);
+            unsafeAtomicAdd(&sum_weight[k], h_weight[k]);
+        }
+    }
+}
+
+// cont_interp_kernel: the new get_mean_cont, scipy's interp1d(x, y,
+// fill_value="extrapolate") (linear, nearest end interval outside [x0, x_{n-1}]), at every
+// pixel's rest-frame log lambda, times the transmission correction
+// (py/picca/delta_extraction/expected_fluxes/utils.py:49-64).  The interval and the slope
+// follow scipy's _call_linear: i = searchsorted(x, v) clipped to [1, n - 1],
+// y = (y_i - y_{i-1}) / (x_i - x_{i-1}) * (v - x_{i-1}) + y_{i-1}.
+__global__ void __launch_bounds__(256)
+cont_interp_kernel(const double *__restrict__ ll_rest, const double *__restrict__ trans,
+                   const double *__restrict__ xs, const double *__restrict__ ys, int32_t n,
+                   double *__restrict__ mean_cont, int64_t npix) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *sx = reinterpret_cast<double *>(smem);
+    double *sy = sx + n;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        sx[k] = xs[k];
+        sy[k] = ys[k];
+    }
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += stride) {
+        const double v = ll_rest[p];
+        // first index with sx[i] >= v (numpy searchsorted, side "left")
+        int lo = 0, hi = n;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (sx[mid] < v) lo = mid + 1;
+            else hi = mid;
+        }
+        const int i = lo < 1 ? 1 : (lo > n - 1 ? n - 1 : lo);
+        const double x_lo = sx[i - 1], x_hi = sx[i];
+        const double y_lo = sy[i - 1], y_hi = sy[i];
+        const double slope = (y_hi - y_lo) / (x_hi - x_lo);
+        mean_cont[p] = (slope * (v - x_lo) + y_lo) * trans[p];
+    }
+}
+
+hipError_t launch_cont_stack(const ContIn &I, const double *a, const double *b,
+                             const int32_t *status, const int32_t *bins, int32_t nbins,
+                             double *sum_ratio, double *sum_weight, int64_t n_forests,
+                             int num_cus, hipStream_t stream) {
+    if (n_forests <= 0 || nbins <= 0) return hipSuccess;
+    const size_t lds = 2 * sizeof(double) * (size_t)nbins;
+    int64_t blocks = (n_forests + 3) / 4;
+    if (blocks > 8 * (int64_t)num_cus) blocks = 8 * (int64_t)num_cus;
+    hipError_t err = hipFuncSetAttribute((const void *)cont_stack_kernel,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(cont_stack_kernel, dim3((unsigned)blocks), dim3(256), lds, stream, I, a,
+                       b, status, bins, nbins, sum_ratio, sum_weight, n_forests);
+    return hipGetLastError();
+}
+
+hipError_t launch_cont_interp(const double *ll_rest, const double *trans, const double *xs,
+                              const double *ys, int32_t n, double *mean_cont, int64_t npix,
+                              int num_cus, hipStream_t stream) {
+    if (npix <= 0) return hipSuccess;
+    const size_t lds = 2 * sizeof(double) * (size_t)n;
+    int64_t blocks = (npix + 255) / 256;
+    if (blocks > 16 * (int64_t)num_cus) blocks = 16 * (int64_t)num_cus;
+    hipError_t err = hipFuncSetAttribute((const void *)cont_interp_kernel,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(cont_interp_kernel, dim3((unsigned)blocks), dim3(256), lds, stream,
+                       ll_rest, trans, xs, ys, n, mean_cont, npix);
+    return hipGetLastError();
+}
+
 }  // namespace picca

@@ -183,6 +183,12 @@ struct picca_ctx {
     DeviceBuffer<int32_t> c_order;
     DeviceBuffer<double> c_a0, c_b0, c_a, c_b, c_chi2;
     DeviceBuffer<int32_t> c_ndata, c_status, c_evals;
+    // DR16 iteration: per pixel rest-frame log lambda, transmission correction, rest-frame
+    // bin; the stacked sums; the mean continuum's interpolation nodes
+    DeviceBuffer<double> c_llr, c_trans, c_stack, c_xs, c_ys;
+    DeviceBuffer<int32_t> c_bin;
+    int32_t cont_nbins = 0;     // 0: no rest frame set for the resident forests
+    bool cont_fitted = false;   // a fit of the resident forests is on the device
 
     // seed network: parameters, Adam state, training data and scratch
     int32_t mlp_in = 0, mlp_hidden = 0;

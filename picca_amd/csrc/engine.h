@@ -428,6 +428,15 @@ struct ContOut {
 
 hipError_t launch_cont_fit(const ContConst &K, const ContIn &I, const ContOut &O,
                            int64_t n_forests, hipStream_t stream);
+// DR16 iteration: weighted F / C stack in rest-frame bins over the forests whose last fit
+// is valid (status 0); interpolation of the new mean continuum to every pixel
+hipError_t launch_cont_stack(const ContIn &I, const double *a, const double *b,
+                             const int32_t *status, const int32_t *bins, int32_t nbins,
+                             double *sum_ratio, double *sum_weight, int64_t n_forests,
+                             int num_cus, hipStream_t stream);
+hipError_t launch_cont_interp(const double *ll_rest, const double *trans, const double *xs,
+                              const double *ys, int32_t n, double *mean_cont, int64_t npix,
+                              int num_cus, hipStream_t stream);
 
 // ---------------------------------------------------------------------------------------
 // seed network of the continuum fit

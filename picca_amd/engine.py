@@ -213,6 +213,34 @@ class Engine:
         out["kernel_ms"] = ms.value
         return out
 
+    def cont_set_rest_frame(self, log_lambda_rest, transmission, bins, num_bins):
+        """Per-pixel rest-frame log lambda, transmission correction and rest-frame bin of
+        the resident continuum forests."""
+        llr = as_f64(log_lambda_rest)
+        trans = as_f64(transmission)
+        bins = np.ascontiguousarray(bins, dtype=np.int32)
+        check(self._lib.picca_cont_set_rest_frame(self._ctx, ptr(llr), ptr(trans), ptr(bins),
+                                                  int(num_bins)), "picca_cont_set_rest_frame")
+        self._cont_nbins = int(num_bins)
+
+    def cont_stack(self, num_bins):
+        """Weighted F / C stack of the last fit: (sum_ratio, sum_weight, kernel_ms)."""
+        total = np.zeros(num_bins)
+        weight = np.zeros(num_bins)
+        ms = ctypes.c_double()
+        check(self._lib.picca_cont_stack(self._ctx, ptr(total), ptr(weight), ctypes.byref(ms)),
+              "picca_cont_stack")
+        return total, weight, ms.value
+
+    def cont_set_mean_cont(self, x, y):
+        """Interpolate the mean continuum nodes to every pixel; returns the kernel ms."""
+        x = as_f64(x)
+        y = as_f64(y)
+        ms = ctypes.c_double()
+        check(self._lib.picca_cont_set_mean_cont(self._ctx, len(x), ptr(x), ptr(y),
+                                                 ctypes.byref(ms)), "picca_cont_set_mean_cont")
+        return ms.value
+
     def diag_copy(self, nbytes):
         """Plain device copy of nbytes (multiple of 16); returns the kernel time in ms."""
         ms = ctypes.c_double()

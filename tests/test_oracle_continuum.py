@@ -10,7 +10,13 @@ synthetic spectra.  What fixes the oracle instead:
     definite, and no point of a dense grid around it is lower;
   * order 0 keeps the slope at 0;
   * stacking noise-free spectra whose flux is exactly proportional to their continuum
-    leaves the mean continuum unchanged (mean of F/C = 1 after normalisation).
+    leaves the mean continuum unchanged (mean of F/C = 1 after normalisation);
+  * a hand-built stack of three tiny forests -- pixels placed on either side of the half
+    step where find_bins_log rounds, weights 1, 3 and 1/3 from chosen ivar, continuum,
+    eta, var_lss and fudge, one forest with a failed fit -- gives the weighted means and
+    the normalisation worked out by hand;
+  * the iteration loop with one iteration is the plain fit, and its re-stack uses the
+    previous mean continuum at the kept grid points.
 """
 import numpy as np
 import pytest
@@ -92,3 +98,73 @@ def test_stack_of_proportional_spectra_is_unchanged(forests):
     kept, new, weight = C.stack_mean_cont(specs, conts, grid, C.find_bins_log, previous)
     np.testing.assert_allclose(new, 1.0, rtol=1e-12)
     assert np.all(weight > 0)
+
+
+def _tiny_spec(z, ll_rest, flux, ivar, eta=1.0, var_lss=0.0, fudge=0.0):
+    n = len(ll_rest)
+    ll = np.asarray(ll_rest) + np.log10(1 + z)
+    return {"log_lambda": ll, "flux": np.asarray(flux, float), "ivar": np.asarray(ivar, float),
+            "eta": np.full(n, eta), "var_lss": np.full(n, var_lss),
+            "fudge": np.full(n, fudge), "z_qso": z}
+
+
+def test_stack_weighted_means_worked_by_hand():
+    step = 1e-4
+    grid = C.rest_frame_grid_log(1040.0, 1200.0, step)
+    k1, k2, k3 = 10, 20, 30
+    # forest 1 (z 2.4): C = 1, ivar = 1, eta 1 -> w = ivar C^2 = 1; F / C = 2 in bins k1, k2
+    # (pixels 0.45 step above grid[k1] and 0.45 step below grid[k2]: both round to them)
+    s1 = _tiny_spec(2.4, [grid[k1] + 0.45 * step, grid[k2] - 0.45 * step], [2.0, 2.0],
+                    [1.0, 1.0])
+    c1 = np.array([1.0, 1.0])
+    # forest 2 (z 2.9): C = 2, ivar = 0.75 -> w = 3; F / C = 6 in bins k1 and k3 (a pixel
+    # 0.55 step above grid[k3 - 1] rounds up to k3)
+    s2 = _tiny_spec(2.9, [grid[k1] - 0.3 * step, grid[k3 - 1] + 0.55 * step], [12.0, 12.0],
+                    [0.75, 0.75])
+    c2 = np.array([2.0, 2.0])
+    # forest 3 (z 3.1): eta 2, var_lss 0.5, fudge 0.5, ivar 1, C = 1 ->
+    # w = 1 / (2 + 0.5 + 0.5) = 1/3; F / C = 9 in bin k2
+    s3 = _tiny_spec(3.1, [grid[k2] + 0.1 * step], [9.0], [1.0], eta=2.0, var_lss=0.5,
+                    fudge=0.5)
+    c3 = np.array([1.0])
+    # forest 4: a failed fit (no continuum): skipped, whatever its pixels hold
+    s4 = _tiny_spec(2.2, [grid[k1], grid[k3]], [1e3, 1e3], [1.0, 1.0])
+    previous = np.linspace(1.0, 2.0, grid.size)
+    kept, new, weight = C.stack_mean_cont([s1, s2, s3, s4], [c1, c2, c3, None], grid,
+                                          C.find_bins_log, previous)
+    np.testing.assert_array_equal(kept, grid[[k1, k2, k3]])
+    # bin k1: (1 * 2 + 3 * 6) / 4 = 5; bin k2: (1 * 2 + 9 / 3) / (4 / 3) = 3.75; bin k3: 6
+    means = np.array([5.0, 3.75, 6.0])
+    np.testing.assert_allclose(weight, [4.0, 4.0 / 3.0, 3.0], rtol=1e-14)
+    expected = previous[[k1, k2, k3]] * means / means.mean()
+    np.testing.assert_allclose(new, expected, rtol=1e-14)
+
+
+def test_find_bins_lin_rounds_at_half_step():
+    grid = np.log10(np.arange(1040.0, 1200.0 + 0.4, 0.8))
+    lam = np.array([1040.0 + 0.8 * 5 + 0.39, 1040.0 + 0.8 * 5 + 0.41, 1040.0 + 0.8 * 7 - 0.39])
+    np.testing.assert_array_equal(C.find_bins_lin(np.log10(lam), grid), [5, 6, 7])
+
+
+def test_iterate_one_iteration_is_the_plain_fit(forests):
+    grid = C.rest_frame_grid_log(1040.0, 1200.0, 1e-4)
+    specs = [dict(s) for s in forests[:5]]
+    fits, x, y, history = C.iterate(specs, grid, 1)
+    assert history == [] and np.array_equal(x, grid) and np.all(y == 1.0)
+    for spec, f in zip(forests[:5], fits):
+        assert f[:4] == C.fit(dict(spec, mean_cont=np.ones_like(spec["flux"])))[:4]
+
+
+def test_iterate_restacks_on_the_previous_mean_continuum(forests):
+    """After two re-stacks the mean continuum at the kept grid points is the product of the
+    two normalised stacks (each stack multiplies the previous interpolated continuum)."""
+    grid = C.rest_frame_grid_log(1040.0, 1200.0, 1e-4)
+    specs = [dict(s) for s in forests[:8]]
+    fits, x, y, history = C.iterate(specs, grid, 3)
+    assert len(history) == 2
+    (x1, y1), (x2, y2) = history
+    assert np.array_equal(x2, x) and np.array_equal(y2, y)
+    # the second stack's ratio to the first continuum (interpolated) is itself normalised
+    ratio = y2 / C.mean_cont_interpolator(x1, y1)(x2)
+    assert abs(ratio.mean() - 1.0) < 1e-12
+    assert all(f[4] for f in fits)
