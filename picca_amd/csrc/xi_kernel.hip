@@ -356,10 +356,11 @@ __device__ void xi_task(const XiConst &C, const XiConst &C_lds, const ForestView
                 T.j_lo = j_lo;
                 T.j_hi = j_hi;
                 T.q0 = q0;
+                T.r_lo = j_hi > j_lo ? uniform_f64(F.rd[ob + j_lo].x) : 0.0;
                 DeviceWalkMem mem{F.rd + ob, F.pre + (ob + b), F.w + ob, F.z + ob, F.dw + ob,
                                   hist, lds_stats, 0u, 0u, 0.0};
                 if (active) {
-                    const WalkLane L{r1, d1, zh1, w1, dw1};
+                    const WalkLane L{r1, d1, zh1, w1, dw1, (lane + 0.5) * (1.0 / 64.0)};
                     walk_lane(C, T, L, mem);
                 }
                 cnt.visit += (uint64_t)mem.cover;

@@ -84,12 +84,14 @@ struct WalkTask {
     double cp, st;       // cos(ang/2) * np / span_p and sin(ang/2) * nt / r_trans_max
     double rc, dc;       // r_c and d_c of delta2's prefix sums
     double q0;           // first estimate of delta2 pixels per unit of the r_par coordinate
+    double r_lo;         // r_comov of delta2's pixel j_lo
     int32_t j_lo, j_hi;  // pruned pixel window of delta2
 };
 
 // Pixel i of delta1 (weight != 0).
 struct WalkLane {
     double r1, d1, zh1, w1, dw1;  // zh1 = z1 / 2, dw1 = delta1 * w1
+    double phase;                 // in [0, 1): where in its r_par range the walk starts
 };
 
 // Key of a pixel pair: m is the signed r_par run index, m = bin for r_comov2 >= r_comov1 and
@@ -218,66 +220,106 @@ PICCA_HD inline void walk_flush(const XiConst &C, const WalkTask &T, const WalkL
     mem.add(row * C.ntp + k.c, L.w1 * dw, L.dw1 * dx, rp, rt, z, (uint32_t)n);
 }
 
-// All pairs of lane pixel L with delta2's window [T.j_lo, T.j_hi).
+// First guess of the end of the run that starts at pixel j with key k: the pixel where the
+// r_par coordinate reaches the run's upper edge at q pixels per unit, in (j, j_end].
+PICCA_HD inline int walk_guess(WalkKey k, double t_j, double q, int j, int j_end) {
+    double ahead = ceil(((double)(k.m + 1) - t_j) * q);
+    ahead = fmin(fmax(ahead, 1.0), (double)(j_end - j));
+    return j + (int)ahead;
+}
+
+PICCA_HD inline bool walk_same(WalkKey a, WalkKey b) { return a.m == b.m && a.c == b.c; }
+
+// r_comov2 beyond r_par_max on the far side, or r_trans beyond r_trans_max: every later
+// pixel is rejected too
+PICCA_HD inline bool walk_terminal(const XiConst &C, WalkKey k) {
+    return k.m >= C.np || k.c >= C.nt;
+}
+
+// All pairs of lane pixel L with delta2's pixels [j_begin, j_end).
+//
+// Every round of the loop tests one candidate end g of the current run [j, ...): it reads
+// pixels g - 1 and g and the prefix entry at g together (one memory round trip), so a run
+// whose end is guessed right closes in one round, and the lane moves on without waiting
+// for the other lanes of its wave.  Pixel g - 1 with the run's key and pixel g without it
+// close the run at g; otherwise the bracket (lo, hi) of the end shrinks: a step back after
+// the first overshoot, then bisection; galloping forward after an undershoot.
 template <class Mem>
-PICCA_HD inline void walk_lane(const XiConst &C, const WalkTask &T, const WalkLane &L, Mem &mem) {
-    const int j_end = T.j_hi;
-    int j = T.j_lo;
+PICCA_HD inline void walk_segment(const XiConst &C, const WalkTask &T, const WalkLane &L,
+                                  Mem &mem, int j_begin, int j_end) {
+    int j = j_begin;
     if (j >= j_end) return;
     double t_j;
     bool near_j;
     WalkKey k = walk_key(C, T, L, mem, j, t_j, near_j);
     PrefixRecord ps = mem.prefix(j);
+    if (near_j) walk_special(C, T, L, mem, j);
+    if (walk_terminal(C, k)) return;
     double q = T.q0;
+    int lo = j, hi = j_end;   // last pixel known to have key k; first pixel known not to
+    bool near_lo = near_j;
+    int g = walk_guess(k, t_j, q, j, j_end);
+    int misses = 0, step = 1;
     for (;;) {
-        if (near_j) walk_special(C, T, L, mem, j);
-        // r_comov2 beyond r_par_max on the far side, or r_trans beyond r_trans_max: every
-        // later pixel is rejected too
-        if (k.m >= C.np || k.c >= C.nt) break;
-        // first pixel after j with another key: hi; last pixel with key k: lo
-        int lo = j, hi = j_end;
-        double t_hi = 0.0;
-        bool near_lo = near_j, near_hi = false;
-        WalkKey k_hi = k;
-        double ahead = ceil(((double)(k.m + 1) - t_j) * q);
-        ahead = fmin(fmax(ahead, 1.0), (double)(j_end - 1 - j));
-        int p = j + (int)ahead;
-        int fwd = 1, back = 0;
-        bool bracketed = false;
-        while (hi - lo > 1) {
-            if (p <= lo) p = lo + 1;
-            if (p >= hi) p = hi - 1;
-            double tp;
-            bool np_;
-            const WalkKey kp = walk_key(C, T, L, mem, p, tp, np_);
-            mem.count_probe();
-            if (kp.m == k.m && kp.c == k.c) {
-                lo = p;
-                near_lo = np_;
-                p = bracketed ? ((lo + hi) >> 1) : lo + fwd;
-                fwd <<= 1;
-            } else {
-                hi = p;
-                t_hi = tp;
-                near_hi = np_;
-                k_hi = kp;
-                p = back < 2 ? hi - 1 : ((lo + hi) >> 1);
-                ++back;
-                bracketed = true;
-            }
+        const bool has1 = g - 1 > lo;    // pixel g - 1 not known yet
+        const bool has2 = g < j_end;
+        double t1 = 0.0, t2 = 0.0;
+        bool n1 = near_lo, n2 = false;
+        WalkKey k1 = k, k2 = k;
+        if (has1) k1 = walk_key(C, T, L, mem, g - 1, t1, n1);
+        if (has2) k2 = walk_key(C, T, L, mem, g, t2, n2);
+        const PrefixRecord pg = mem.prefix(g);
+        mem.count_probe();
+        if (!walk_same(k1, k)) {          // the run ends at g - 1 or before
+            hi = g - 1;
+            g = misses == 0 ? hi : lo + 1 + ((hi - lo) >> 1);
+            ++misses;
+            continue;
         }
-        if (lo > j && near_lo) walk_special(C, T, L, mem, lo);
-        const PrefixRecord pe = mem.prefix(hi);
-        mem.count_run(pe.c - ps.c);
-        if (k.m >= -C.np) walk_flush(C, T, L, mem, k, ps, pe);
-        if (hi >= j_end) break;
-        if (hi - j >= 3 && t_hi > t_j) q = (double)(hi - j) / (t_hi - t_j);
-        j = hi;
-        k = k_hi;
-        t_j = t_hi;
-        near_j = near_hi;
-        ps = pe;
+        if (has2 && walk_same(k2, k)) {   // the run goes on past g
+            lo = g;
+            near_lo = n2;
+            g = lo + step < hi ? lo + step : hi;
+            step <<= 1;
+            continue;
+        }
+        // the run is [j, g)
+        if (g - 1 > j && n1) walk_special(C, T, L, mem, g - 1);
+        mem.count_run(pg.c - ps.c);
+        if (k.m >= -C.np) walk_flush(C, T, L, mem, k, ps, pg);
+        if (!has2) break;
+        if (g - j >= 3 && t2 > t_j) q = (double)(g - j) / (t2 - t_j);
+        j = g;
+        k = k2;
+        t_j = t2;
+        near_j = n2;
+        ps = pg;
+        if (near_j) walk_special(C, T, L, mem, j);
+        if (walk_terminal(C, k)) break;
+        lo = j;
+        hi = j_end;
+        near_lo = near_j;
+        misses = 0;
+        step = 1;
+        g = walk_guess(k, t_j, q, j, j_end);
     }
+}
+
+// All pairs of lane pixel L with delta2's window [T.j_lo, T.j_hi), walked as
+// [s, j_hi) then [j_lo, s).  Every lane meets the bins in the same order along j, and the
+// lanes of a wave move at about the same pace, so walks that all started at j_lo would
+// add to one bin at a time and every LDS atomic would meet 64 lanes on one address.  The
+// split point s sits at the fraction `phase` of the lane's r_par range
+// [r1 - r_par_max / cos, r1 + r_par_max / cos] (placed with the window's pixel density;
+// any s is exact), so the lanes of a wave start in different bins.
+template <class Mem>
+PICCA_HD inline void walk_lane(const XiConst &C, const WalkTask &T, const WalkLane &L, Mem &mem) {
+    const double r_start = L.r1 + (2.0 * L.phase - 1.0) * (C.np_d / T.cp);
+    double s = (double)T.j_lo + (r_start - T.r_lo) * T.cp * T.q0;
+    s = fmin(fmax(s, (double)T.j_lo), (double)T.j_hi);
+    const int split = (int)s;
+    walk_segment(C, T, L, mem, split, T.j_hi);
+    walk_segment(C, T, L, mem, T.j_lo, split);
 }
 
 }  // namespace picca

@@ -8,8 +8,9 @@ Migrad's start (mean flux, slope 0); the plain fit is the judge: the seeded fit 
 at the same minimum chi2.
 
 Training and inference run on the GPU (``picca_mlp_*`` in the C-ABI): BF16 matrix-core
-products with FP32 accumulation, Adam in FP32.  Targets are the plain fit's (a, b) on a
-training set of spectra.
+products with FP32 accumulation, Adam in FP32.  Targets are the (a, b) of the last
+iteration of picca's DR16 loop (``continuum.compute_expected_flux``) on a training set of
+spectra.
 """
 import ctypes
 

@@ -227,6 +227,7 @@ int xi_walk_host(const picca_cf_params *params, int64_t num_forests, const int64
                 T.j_lo = j_lo;
                 T.j_hi = j_hi;
                 T.q0 = q0;
+                T.r_lo = j_hi > j_lo ? r[ob + j_lo] : 0.0;
                 HostMem mem;
                 mem.rp_ = r + ob;
                 mem.dp_ = d + ob;
@@ -240,7 +241,7 @@ int xi_walk_host(const picca_cf_params *params, int64_t num_forests, const int64
                 for (int i = i0; i < i1; ++i) {
                     if (w[oa + i] == 0) continue;
                     const picca::WalkLane L{r[oa + i], d[oa + i], z[oa + i] * 0.5, w[oa + i],
-                                            dw[oa + i]};
+                                            dw[oa + i], (i - i0 + 0.5) * (1.0 / 64.0)};
                     picca::walk_lane(C, T, L, mem);
                 }
             } else {

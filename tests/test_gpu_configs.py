@@ -18,7 +18,7 @@ import pytest
 
 import walk_cases as W
 from oracle import cf_oracle as O
-from picca_amd import batch, synth
+from picca_amd import _native, batch, synth
 from picca_amd.constants import Cosmo
 from picca_amd.engine import get_engine, params_from_globals
 from picca_amd.neighbours import find_neighbours
@@ -47,9 +47,9 @@ def sampled_pairs(survey, n_sources, seed, r_max=200.0):
                          row_of_src[s_of].astype(np.int32), uniq)
 
 
-def run_and_check(store, pairs, g=None):
+def run_and_check(store, pairs, g=None, schedule=None):
     g = g or W.Globals()
-    res = batch.run_xi(store, pairs, params_from_globals(g))
+    res = batch.run_xi(store, pairs, params_from_globals(g), schedule=schedule)
     ref, ost = O.xi_rows(O.params_from_module(g), store, pairs.pair_a, pairs.pair_b,
                          pairs.pair_ang, pairs.pair_flags, pairs.pair_row, pairs.num_rows,
                          num_threads=16)
@@ -73,12 +73,13 @@ def test_config0_full_catalogue():
 
 @pytest.mark.parametrize("name", ["survey_100k", "survey_300k", "survey_100k_heavy"])
 def test_full_size_surveys(name):
+    """Both schedules of the pair kernel: the default (automatic) and the run walk, which
+    takes every task whose pruned window is not empty on this grid."""
     s = synth.make_survey(name, seed=0, scale=1.0)
     pairs = sampled_pairs(s, 15, seed=11)
     assert 10_000 < pairs.num_pairs < 60_000
-    res, ref, ost = run_and_check(s.store, pairs)
-    # the 0.8 A grid holds about eight pixels per bin: the run walk takes the tasks whose
-    # pruned window is not empty
+    run_and_check(s.store, pairs)
+    res, _, _ = run_and_check(s.store, pairs, schedule=_native.XI_SCHEDULE_RUNS)
     rep = res.report
     assert rep["walk_tasks"] > 0.9 * (rep["tasks"] - rep["empty_tasks"])
 

@@ -54,9 +54,7 @@ def test_log_grid_survey():
 
 def test_linear_grid_survey_slice():
     reports = check_case(W.survey_case("survey_100k", 0.001))
-    # the automatic schedule walks the tasks whose pruned window holds enough pixels per
-    # bin (short windows at forest ends keep the pair loop)
-    assert reports["runs"]["walk_tasks"] > reports["auto"]["walk_tasks"] > 1000
+    assert reports["runs"]["walk_tasks"] > 1000
 
 
 @pytest.mark.parametrize("case", ["cf", "cf_image"])

@@ -26,11 +26,12 @@ _lib = None
 def build():
     rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
     hipcc = os.path.join(rocm, "bin", "hipcc")
+    # a per-process temporary name: parallel test workers may build at the same time
     cmd = [hipcc if os.path.exists(hipcc) else "hipcc", "--offload-arch=gfx950", "-O2",
            "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-Wall",
-           "-o", LIBRARY + ".tmp", SOURCE]
+           "-o", LIBRARY + ".tmp%d" % os.getpid(), SOURCE]
     subprocess.run(cmd, check=True)
-    os.replace(LIBRARY + ".tmp", LIBRARY)
+    os.replace(LIBRARY + ".tmp%d" % os.getpid(), LIBRARY)
     return LIBRARY
 
 
