@@ -294,7 +294,9 @@ def bench_xi(args):
                 "traffic": args.traffic_bytes if args.traffic_bytes > 0 else None,
                 "flops_per_launch": flops,
                 "executed_tflops": executed / (kernel_ms / 1e3) / 1e12,
-                "kernel": "picca::xi_kernel<0,false,false,true>", "kernel_ms": kernel_ms,
+                "kernel": "picca::xi_kernel<0,false,false,true,%s>"
+                          % ("true" if args.schedule == "runs" else "false"),
+                "kernel_ms": kernel_ms,
                 "hbm_gbs_at_traffic": (args.traffic_bytes / (kernel_ms / 1e3) / 1e9
                                        if args.traffic_bytes > 0 else None)}
 
@@ -841,72 +843,99 @@ def continuum_sets(n, seed, chunk=20000):
 
 
 def bench_continuum(args):
-    """Continuum fits of picca's first iteration (mean continuum 1, eta 1, var_lss 0.1) on
-    synthetic spectra: the plain GPU fit (Migrad's start) against the fit started from the
-    seed network's prediction, which is trained on the plain fit of another set."""
+    """Config 4: continuum fitting of synthetic quasar spectra with picca's DR16 iteration.
+
+    Both sets run the iteration of Dr16ExpectedFlux.compute_expected_flux on the GPU (5
+    iterations: batched fit, weighted F / C stack, new mean continuum interpolated to every
+    pixel; eta 1, var_lss 0.1, fudge 0 held).  The seed network is trained on the last
+    iteration's (a, b) of the training set.  On the test set, with its own final mean
+    continuum resident, the last iteration's fit runs from Migrad's start (plain) and from
+    the network's prediction (seeded), each under two stopping rules: EDM < 1e-5 (Migrad's
+    kind of rule) and EDM < 1e-4 (the final chi2 within about 1e-4 of the minimum); every
+    final chi2 is compared with the converged one (EDM < 1e-12 chi2)."""
     from picca_amd import continuum, seed_mlp
-    from picca_amd.engine import device_info
+    from picca_amd.engine import device_info, get_engine
 
     n = args.spectra
+    iterations = 5
+    grid = continuum.rest_frame_grid_log(1040.0, 1200.0, 1e-4)
     t0 = time.time()
     train_in, train_x, train_scale = continuum_sets(n // 2, seed=args.seed + 1)
     test_in, test_x, test_scale = continuum_sets(n, seed=args.seed)
     log("continuum: %d training and %d test spectra, %d + %d forest pixels (%.1f s)"
         % (n // 2, n, train_in["offsets"][-1], test_in["offsets"][-1], time.time() - t0))
     info = device_info(0)
-    train_fit = continuum.fit_arrays(train_in, order=1)
-    ok = train_fit["status"] == 0
+    train = continuum.compute_expected_flux(train_in, grid, iterations, order=1)
+    ok = train["fit"]["status"] == 0
     net = seed_mlp.SeedNetwork(seed=args.seed)
     t1 = time.time()
-    losses = net.train(train_x[ok], seed_mlp.targets(train_fit["a"][ok], train_fit["b"][ok],
-                                                    train_scale[ok]),
+    losses = net.train(train_x[ok], seed_mlp.targets(train["fit"]["a"][ok],
+                                                    train["fit"]["b"][ok], train_scale[ok]),
                        epochs=30, batch=512, lr=1e-3)
     train_s = time.time() - t1
     log("seed network: %d Adam steps in %.1f s, loss %.3e -> %.3e"
         % (len(losses), train_s, losses[:20].mean(), losses[-20:].mean()))
-    # both fits stop at Migrad's kind of rule, an absolute EDM bound (here 1e-5, so the
-    # final chi2 is within ~1e-5 of the minimum); the reference minimum comes from the
-    # plain fit run to EDM < 1e-12 chi2
-    exact = continuum.fit_arrays(test_in, order=1)
-    plain_ms, seeded_ms, infer_ms = [], [], []
-    plain = seeded = None
-    for k in range(args.warmup + args.steps):
-        plain = continuum.fit_arrays(test_in, order=1, tol=1e-15, tol_abs=1e-5)
-        y, inf_ms = net.predict(test_x, timing=True)
-        a0, b0 = seed_mlp.seeds(y, test_scale)
-        seeded = continuum.fit_arrays(test_in, order=1, a0=a0, b0=b0, tol=1e-15, tol_abs=1e-5)
-        if k >= args.warmup:
-            plain_ms.append(plain["kernel_ms"])
-            seeded_ms.append(seeded["kernel_ms"])
-            infer_ms.append(inf_ms)
-        log("step %d: plain fit %.3f ms, seeded fit %.3f ms + inference %.3f ms (kernels)"
-            % (k, plain["kernel_ms"], seeded["kernel_ms"], inf_ms))
-    both = (plain["status"] == 0) & (seeded["status"] == 0) & (exact["status"] == 0)
-    dchi = np.abs(seeded["chi2"][both] - exact["chi2"][both])
-    dchi_plain = np.abs(plain["chi2"][both] - exact["chi2"][both])
-    p_ms = float(np.mean(plain_ms))
-    s_ms = float(np.mean(seeded_ms)) + float(np.mean(infer_ms))
+    # the test set's own DR16 iteration; its last fit, converged, is the reference minimum
+    t2 = time.time()
+    test = continuum.compute_expected_flux(test_in, grid, iterations, order=1)
+    dr16_wall_s = time.time() - t2
+    exact = test["fit"]
+    log("DR16 iteration of the test set: %d iterations, %.1f ms of kernels, %.1f s wall"
+        % (iterations, test["kernel_ms"], dr16_wall_s))
+    eng = get_engine(0)     # holds the test forests and their final mean continuum
+    a0, b0 = continuum.start_values(test_in["offsets"], test_in["flux"], test_in["ivar"])
+    rules = {}
+    for rule, tol_abs in (("edm_1e-5", 1e-5), ("edm_1e-4", 1e-4)):
+        plain_ms, seeded_ms, infer_ms = [], [], []
+        plain = seeded = None
+        for k in range(args.warmup + args.steps):
+            plain = eng.cont_fit(a0, b0, tol=1e-15, tol_abs=tol_abs)
+            y, inf_ms = net.predict(test_x, timing=True)
+            sa, sb = seed_mlp.seeds(y, test_scale)
+            seeded = eng.cont_fit(sa, sb, tol=1e-15, tol_abs=tol_abs)
+            if k >= args.warmup:
+                plain_ms.append(plain["kernel_ms"])
+                seeded_ms.append(seeded["kernel_ms"])
+                infer_ms.append(inf_ms)
+        both = (plain["status"] == 0) & (seeded["status"] == 0) & (exact["status"] == 0)
+        d_seeded = np.abs(seeded["chi2"][both] - exact["chi2"][both])
+        d_plain = np.abs(plain["chi2"][both] - exact["chi2"][both])
+        p_ms = float(np.mean(plain_ms))
+        s_ms = float(np.mean(seeded_ms)) + float(np.mean(infer_ms))
+        rules[rule] = {
+            "plain_fit_ms": p_ms, "seeded_fit_plus_inference_ms": s_ms,
+            "inference_kernel_ms": float(np.mean(infer_ms)),
+            "seeded_speedup": p_ms / s_ms,
+            "plain_fits_per_s": n / (p_ms / 1e3), "seeded_fits_per_s": n / (s_ms / 1e3),
+            "plain_mean_passes": float(plain["evaluations"].mean()),
+            "seeded_mean_passes": float(seeded["evaluations"].mean()),
+            "fits_valid_both": int(both.sum()),
+            "chi2_plain_minus_min_max": float(d_plain.max()) if d_plain.size else 0.0,
+            "chi2_seeded_minus_min_max": float(d_seeded.max()) if d_seeded.size else 0.0,
+            "frac_plain_within_1e-4": float((d_plain < 1e-4).mean()) if d_plain.size else 1.0,
+            "frac_seeded_within_1e-4": float((d_seeded < 1e-4).mean()) if d_seeded.size else 1.0,
+        }
+        log("%s: plain %.3f ms (%.2f passes), seeded %.3f ms (%.2f passes) -> %.2fx"
+            % (rule, p_ms, rules[rule]["plain_mean_passes"], s_ms,
+               rules[rule]["seeded_mean_passes"], p_ms / s_ms))
+    main_rule = rules["edm_1e-4"]
     result = {
         "metric": "continuum fits/s (1 GPU), learned seed vs plain fit",
-        "value": n / (s_ms / 1e3), "unit": "fits/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": s_ms, "higher_is_better": True,
+        "value": main_rule["seeded_fits_per_s"], "unit": "fits/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": main_rule["seeded_fit_plus_inference_ms"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64 fit, bf16/fp32 network",
         "data": "synthetic",
         "config": {"workload": "continuum_%d" % n, "spectra": n, "training_spectra": n // 2,
                    "forest_pixels": int(test_in["offsets"][-1]),
-                   "stopping": "EDM < 1e-5 (absolute) for both fits",
-                   "plain_fit_ms": p_ms, "seeded_fit_plus_inference_ms": s_ms,
-                   "inference_kernel_ms": float(np.mean(infer_ms)),
-                   "plain_fits_per_s": n / (p_ms / 1e3),
-                   "seeded_speedup": p_ms / s_ms,
+                   "dr16_iterations": iterations,
+                   "dr16_iteration_kernel_ms": test["kernel_ms"],
+                   "dr16_iteration_wall_s": dr16_wall_s,
+                   "rest_frame_bins_kept": int(len(test["x"])),
+                   "value_rule": "EDM < 1e-4 (final chi2 within ~1e-4 of the minimum)",
+                   "rules": rules,
                    "exact_fit_ms": exact["kernel_ms"],
-                   "exact_mean_evaluations": float(exact["evaluations"].mean()),
-                   "plain_mean_evaluations": float(plain["evaluations"].mean()),
-                   "seeded_mean_evaluations": float(seeded["evaluations"].mean()),
-                   "fits_valid_both": int(both.sum()),
-                   "chi2_seeded_minus_min_max": float(dchi.max()) if dchi.size else 0.0,
-                   "chi2_plain_minus_min_max": float(dchi_plain.max()) if dchi.size else 0.0,
-                   "chi2_frac_within_1e-4": float((dchi < 1e-4).mean()) if dchi.size else 1.0,
+                   "exact_mean_passes": float(exact["evaluations"].mean()),
                    "training_steps": int(len(losses)), "training_s": train_s,
                    "device": info["name"]},
         "cpu_baseline": None,
@@ -915,15 +944,22 @@ def bench_continuum(args):
         from oracle import cont_oracle as C
         from picca_amd import synth_spectra as S
         m = min(n, 300)
+        get_mean_cont = C.mean_cont_interpolator(test["x"], test["y"])
+        specs = []
+        for k in range(m):
+            spec = S.forest_spec(test_in, k)
+            spec["mean_cont"] = get_mean_cont(spec["log_lambda"] - np.log10(1 + spec["z_qso"]))
+            specs.append(spec)
         tc = time.time()
-        res = [C.fit(S.forest_spec(test_in, k), order=1) for k in range(m)]
+        res = [C.fit(spec, order=1) for spec in specs]
         cpu_s = time.time() - tc
         ref = np.array([r[2] for r in res])
         rel = np.abs(exact["chi2"][:m] - ref) / np.abs(ref)
         result["cpu_baseline"] = {
             "value": m / cpu_s, "unit": "fits/s", "cores": 1, "kind": "port",
-            "sample": "first %d test spectra, FP64 numpy objective minimised by scipy "
-                      "Nelder-Mead + Newton polish (iminuit is not installed)" % m,
+            "sample": "first %d test spectra with the final mean continuum, FP64 numpy "
+                      "objective minimised by scipy Nelder-Mead + Newton polish (iminuit "
+                      "is not installed)" % m,
             "seconds": cpu_s}
         result["parity_sample"] = {"chi2_max_rel_err_vs_oracle": float(rel.max())}
     print(json.dumps(result))

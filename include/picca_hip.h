@@ -157,8 +157,9 @@ int picca_xi_execute_noalloc(picca_ctx *ctx, const picca_cf_params *params, void
  * r_par_min == 0, no cuts).  RUNS walks each pixel of delta1 through runs of pixels of
  * delta2 that share one bin, summed from per-forest prefix sums, wherever delta2's pixels
  * increase strictly and are spaced well above the rounding margins; PAIRS evaluates every
- * pixel pair.  AUTO (the default) is RUNS.  Results are identical up to FP64 summation
- * order; no counterpart in picca.
+ * pixel pair.  AUTO (the default) is PAIRS, which measured faster on the synthetic surveys'
+ * grids (runs of 2 and 8 pixels).  Results are identical up to FP64 summation order; no
+ * counterpart in picca.
  */
 #define PICCA_XI_SCHEDULE_AUTO 0
 #define PICCA_XI_SCHEDULE_PAIRS 1

@@ -19,7 +19,7 @@ enum : int {
     XI_HAS_ZERR = 4,
     XI_REMOVE_SHP = 8,
     XI_PRUNE = 16,
-    XI_WALK = 32,   // run walk allowed for eligible tasks of the specialised kernel
+    XI_WALK = 32,   // run walk for the eligible tasks of the specialised kernel
 };
 
 enum : int { PAIR_SAME_HALF_PLATE = 1, PAIR_XCF_COUNT = 2 };
@@ -60,9 +60,6 @@ struct XiConst {
     // r_trans row window of the cross-correlation kernel's LDS histogram: rows
     // [win_t0, win_t0 + win_rows) of the output, padded to (np + 2) x (win_rows + 2) bins
     int32_t win_t0, win_rows, win_last, win_pad_;
-    // run walk of the xi kernel: taken for a task whose delta2 holds at least this many
-    // pixels per r_par bin width (pixel pairs per run), where it beats the pair loop
-    double walk_min_q;
 };
 
 // One pixel as the pair kernel streams it through the scalar cache.

@@ -82,21 +82,13 @@ inline const char *widen_xi_margins(XiConst &C, double max_abs_r, double max_abs
     return nullptr;
 }
 
-// Pixel pairs per run below which the pair loop beats the run walk (pixels of delta2 per
-// r_par bin width).  Measured on the 0.8 A grid of the 100k survey (about 8 pixels per
-// bin) the walk took 47.9 s against the pair loop's 31.1 s, and 12.4 ms against 4.6 ms on
-// the log10-lambda grid (about 2 per bin): the automatic schedule keeps the pair loop.
-constexpr double XI_WALK_MIN_PIXELS_PER_BIN = 1e30;
-
-// Schedule of the specialised kernel (PICCA_XI_SCHEDULE_*): sets XI_WALK and the walk's
-// threshold.
+// Schedule of the specialised kernel (PICCA_XI_SCHEDULE_*).  The automatic choice is the
+// pair loop: on the 0.8 A grid of the 100k survey (about 8 pixel pairs per run) the run walk
+// measured 36.6 s against the pair loop's 31.1 s, and 12.4 ms against 4.6 ms on the
+// log10-lambda grid (about 2 pairs per run).
 inline void apply_xi_schedule(XiConst &C, int schedule) {
-    if (schedule == PICCA_XI_SCHEDULE_PAIRS) {
-        C.flags &= ~XI_WALK;
-        return;
-    }
-    C.flags |= XI_WALK;
-    C.walk_min_q = schedule == PICCA_XI_SCHEDULE_RUNS ? 0.0 : XI_WALK_MIN_PIXELS_PER_BIN;
+    if (schedule == PICCA_XI_SCHEDULE_RUNS) C.flags |= XI_WALK;
+    else C.flags &= ~XI_WALK;
 }
 
 // The specialised kernel (and its run walk) covers standard binning, auto-correlation,

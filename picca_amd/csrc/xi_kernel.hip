@@ -249,7 +249,7 @@ struct DeviceWalkMem {
 //   same-half-plate cuts (the cuts of the reference that cannot trigger are compiled out).
 //   LDS_HIST: the histogram is the padded LDS layout (row stride nt + 1), otherwise the
 //   output row in HBM.
-template <int MODE, bool XCORR, bool GENERAL, bool LDS_HIST, typename Bin>
+template <int MODE, bool XCORR, bool GENERAL, bool LDS_HIST, bool WALK, typename Bin>
 __device__ void xi_task(const XiConst &C, const XiConst &C_lds, const ForestView &F,
                         const PairView &P, int64_t p, int chunk, int lane, Bin *hist,
                         TilePixel *tile, TileExact *tile_x, LaneCounters &cnt,
@@ -330,7 +330,7 @@ __device__ void xi_task(const XiConst &C, const XiConst &C_lds, const ForestView
     j_hi = uniform_int(j_hi);
 
     if constexpr (!GENERAL && LDS_HIST) {
-        if ((C.flags & XI_WALK) && pruned) {
+        if (WALK && (C.flags & XI_WALK) && pruned) {
             // Run walk (xi_walk.h) when delta2's pixels are strictly increasing in r_comov
             // and dist_m and consecutive pixels lie more than four rounding margins apart
             // in both bin coordinates; otherwise the pair loop below.
@@ -345,7 +345,7 @@ __device__ void xi_task(const XiConst &C, const XiConst &C_lds, const ForestView
                 if (span > 0) q0 = (double)(j_hi - 1 - j_lo) / span;
             }
             if (spc_r > 0 && spc_d > 0 && cp * spc_r > 4.0 * (0.5 - C.half_p) &&
-                st * spc_d > 4.0 * (0.5 - C.half_t) && q0 >= C.walk_min_q) {
+                st * spc_d > 4.0 * (0.5 - C.half_t)) {
                 WalkTask T;
                 T.cos_half = cos_half;
                 T.sin_half = sin_half;
@@ -557,7 +557,7 @@ __device__ __forceinline__ uint64_t flush_row(const XiConst &C, LocalBin *hist, 
     return moved;
 }
 
-template <int MODE, bool XCORR, bool GENERAL, bool LDS_HIST>
+template <int MODE, bool XCORR, bool GENERAL, bool LDS_HIST, bool WALK>
 __global__ void __launch_bounds__(XI_BLOCK)
 xi_kernel(XiConst C, ForestView F, PairView P, GroupView G, XiOut O) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -646,10 +646,10 @@ xi_kernel(XiConst C, ForestView F, PairView P, GroupView G, XiOut O) {
             const int chunk = uniform_int(t - s_prefix[q]);
             const uint64_t steps_before = n_steps;
             if (LDS_HIST) {
-                xi_task<MODE, XCORR, GENERAL, true>(C, s_const, F, P, p0 + q, chunk, lane, hist,
+                xi_task<MODE, XCORR, GENERAL, true, WALK>(C, s_const, F, P, p0 + q, chunk, lane, hist,
                                                     tile, tile_x, cnt, lds_stats, n_steps, n_walk);
             } else {
-                xi_task<MODE, XCORR, GENERAL, false>(C, s_const, F, P, p0 + q, chunk, lane,
+                xi_task<MODE, XCORR, GENERAL, false, false>(C, s_const, F, P, p0 + q, chunk, lane,
                                                      O.bins + (int64_t)row * C.nb, tile, tile_x,
                                                      cnt, lds_stats, n_steps, n_walk);
             }
@@ -726,11 +726,11 @@ hipError_t launch_zero_u64(unsigned long long *ptr, int64_t n, hipStream_t strea
     return hipGetLastError();
 }
 
-template <int MODE, bool XCORR, bool GENERAL, bool LDS_HIST>
+template <int MODE, bool XCORR, bool GENERAL, bool LDS_HIST, bool WALK>
 static hipError_t launch_xi_t(const XiConst &C, const ForestView &F, const PairView &P,
                               const GroupView &G, const XiOut &O, int grid, size_t lds,
                               hipStream_t stream) {
-    auto kern = xi_kernel<MODE, XCORR, GENERAL, LDS_HIST>;
+    auto kern = xi_kernel<MODE, XCORR, GENERAL, LDS_HIST, WALK>;
     hipError_t err = hipFuncSetAttribute((const void *)kern,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (err != hipSuccess) return err;
@@ -742,18 +742,24 @@ template <int MODE, bool XCORR, bool GENERAL>
 static hipError_t launch_xi_h(bool lds_hist, const XiConst &C, const ForestView &F,
                               const PairView &P, const GroupView &G, const XiOut &O, int grid,
                               size_t lds, hipStream_t stream) {
-    return lds_hist ? launch_xi_t<MODE, XCORR, GENERAL, true>(C, F, P, G, O, grid, lds, stream)
-                    : launch_xi_t<MODE, XCORR, GENERAL, false>(C, F, P, G, O, grid, lds, stream);
+    return lds_hist
+               ? launch_xi_t<MODE, XCORR, GENERAL, true, false>(C, F, P, G, O, grid, lds, stream)
+               : launch_xi_t<MODE, XCORR, GENERAL, false, false>(C, F, P, G, O, grid, lds, stream);
 }
 
 // general == false is only valid for MODE_STANDARD, auto-correlation, r_par_min == 0 and
-// no z / zerr / same-half-plate cuts (the host decides).
+// no z / zerr / same-half-plate cuts (the host decides).  The run walk (XI_WALK) has a
+// kernel of its own, so that its registers do not weigh on the pair loop's.
 hipError_t launch_xi(int mode, bool xcorr, bool general, bool lds_hist, const XiConst &C,
                      const ForestView &F, const PairView &P, const GroupView &G,
                      const XiOut &O, int grid, size_t lds, hipStream_t stream) {
     if (mode == MODE_STANDARD) {
-        if (!general && !xcorr)
+        if (!general && !xcorr) {
+            if (lds_hist && (C.flags & XI_WALK))
+                return launch_xi_t<MODE_STANDARD, false, false, true, true>(C, F, P, G, O, grid,
+                                                                            lds, stream);
             return launch_xi_h<MODE_STANDARD, false, false>(lds_hist, C, F, P, G, O, grid, lds, stream);
+        }
         return xcorr ? launch_xi_h<MODE_STANDARD, true, true>(lds_hist, C, F, P, G, O, grid, lds, stream)
                      : launch_xi_h<MODE_STANDARD, false, true>(lds_hist, C, F, P, G, O, grid, lds, stream);
     }

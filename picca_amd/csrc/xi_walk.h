@@ -236,90 +236,114 @@ PICCA_HD inline bool walk_terminal(const XiConst &C, WalkKey k) {
     return k.m >= C.np || k.c >= C.nt;
 }
 
-// All pairs of lane pixel L with delta2's pixels [j_begin, j_end).
+// All pairs of lane pixel L with delta2's window [T.j_lo, T.j_hi).
 //
-// Every round of the loop tests one candidate end g of the current run [j, ...): it reads
-// pixels g - 1 and g and the prefix entry at g together (one memory round trip), so a run
-// whose end is guessed right closes in one round, and the lane moves on without waiting
-// for the other lanes of its wave.  Pixel g - 1 with the run's key and pixel g without it
-// close the run at g; otherwise the bracket (lo, hi) of the end shrinks: a step back after
-// the first overshoot, then bisection; galloping forward after an undershoot.
-template <class Mem>
-PICCA_HD inline void walk_segment(const XiConst &C, const WalkTask &T, const WalkLane &L,
-                                  Mem &mem, int j_begin, int j_end) {
-    int j = j_begin;
-    if (j >= j_end) return;
-    double t_j;
-    bool near_j;
-    WalkKey k = walk_key(C, T, L, mem, j, t_j, near_j);
-    PrefixRecord ps = mem.prefix(j);
-    if (near_j) walk_special(C, T, L, mem, j);
-    if (walk_terminal(C, k)) return;
-    double q = T.q0;
-    int lo = j, hi = j_end;   // last pixel known to have key k; first pixel known not to
-    bool near_lo = near_j;
-    int g = walk_guess(k, t_j, q, j, j_end);
-    int misses = 0, step = 1;
-    for (;;) {
-        const bool has1 = g - 1 > lo;    // pixel g - 1 not known yet
-        const bool has2 = g < j_end;
-        double t1 = 0.0, t2 = 0.0;
-        bool n1 = near_lo, n2 = false;
-        WalkKey k1 = k, k2 = k;
-        if (has1) k1 = walk_key(C, T, L, mem, g - 1, t1, n1);
-        if (has2) k2 = walk_key(C, T, L, mem, g, t2, n2);
-        const PrefixRecord pg = mem.prefix(g);
-        mem.count_probe();
-        if (!walk_same(k1, k)) {          // the run ends at g - 1 or before
-            hi = g - 1;
-            g = misses == 0 ? hi : lo + 1 + ((hi - lo) >> 1);
-            ++misses;
-            continue;
-        }
-        if (has2 && walk_same(k2, k)) {   // the run goes on past g
-            lo = g;
-            near_lo = n2;
-            g = lo + step < hi ? lo + step : hi;
-            step <<= 1;
-            continue;
-        }
-        // the run is [j, g)
-        if (g - 1 > j && n1) walk_special(C, T, L, mem, g - 1);
-        mem.count_run(pg.c - ps.c);
-        if (k.m >= -C.np) walk_flush(C, T, L, mem, k, ps, pg);
-        if (!has2) break;
-        if (g - j >= 3 && t2 > t_j) q = (double)(g - j) / (t2 - t_j);
-        j = g;
-        k = k2;
-        t_j = t2;
-        near_j = n2;
-        ps = pg;
-        if (near_j) walk_special(C, T, L, mem, j);
-        if (walk_terminal(C, k)) break;
-        lo = j;
-        hi = j_end;
-        near_lo = near_j;
-        misses = 0;
-        step = 1;
-        g = walk_guess(k, t_j, q, j, j_end);
-    }
-}
-
-// All pairs of lane pixel L with delta2's window [T.j_lo, T.j_hi), walked as
-// [s, j_hi) then [j_lo, s).  Every lane meets the bins in the same order along j, and the
-// lanes of a wave move at about the same pace, so walks that all started at j_lo would
-// add to one bin at a time and every LDS atomic would meet 64 lanes on one address.  The
-// split point s sits at the fraction `phase` of the lane's r_par range
-// [r1 - r_par_max / cos, r1 + r_par_max / cos] (placed with the window's pixel density;
-// any s is exact), so the lanes of a wave start in different bins.
+// The window is walked as [s, j_hi) then [j_lo, s).  Every lane meets the bins in the same
+// order along j and the lanes of a wave move at about the same pace, so walks that all
+// began at j_lo would add to a few neighbouring bins at a time; the split point s sits at
+// the fraction `phase` of the lane's r_par range [r1 - r_par_max / cos, r1 + r_par_max /
+// cos] (placed with the window's pixel density; any s is exact), so the lanes of a wave
+// add to different bins.
+//
+// One loop, one memory round trip per pass: every pass loads pixels pa, pb (their r_comov
+// and dist_m) and the prefix entry at pb, evaluates both keys, and then updates the lane's
+// state with short selects, so the lanes of a wave stay on one instruction stream whatever
+// state each is in:
+//   * start of a segment at j (pa = pb = j): the run's key, its prefix entry, the first
+//     guess of its end;
+//   * test of a candidate end g of the run [j, ...) (pa = g - 1, pb = g): pixel g - 1 with
+//     the run's key and pixel g without it close the run at g; otherwise the bracket
+//     (lo, hi) of the end shrinks (a step back after the first overshoot, then bisection;
+//     galloping forward after an undershoot).
+// A run whose end is guessed right closes in one pass.
 template <class Mem>
 PICCA_HD inline void walk_lane(const XiConst &C, const WalkTask &T, const WalkLane &L, Mem &mem) {
+    if (T.j_lo >= T.j_hi) return;
     const double r_start = L.r1 + (2.0 * L.phase - 1.0) * (C.np_d / T.cp);
     double s = (double)T.j_lo + (r_start - T.r_lo) * T.cp * T.q0;
     s = fmin(fmax(s, (double)T.j_lo), (double)T.j_hi);
     const int split = (int)s;
-    walk_segment(C, T, L, mem, split, T.j_hi);
-    walk_segment(C, T, L, mem, T.j_lo, split);
+    // first segment [split, j_hi); an empty one goes straight to [j_lo, split)
+    bool second = split >= T.j_hi;
+    int seg_end = second ? split : T.j_hi;
+    int j = second ? T.j_lo : split;
+    bool active = j < seg_end;
+    bool starting = true;
+    WalkKey k = {0, 0};
+    double t_j = 0.0, q = T.q0;
+    bool near_j = false, near_lo = false;
+    PrefixRecord ps = {0, 0, 0, 0, 0, 0};
+    int lo = j, hi = seg_end, g = j, misses = 0, step = 1;
+    const int last = T.j_hi > 0 ? T.j_hi - 1 : 0;   // pixel reads stay inside the window
+    while (active) {
+        const int pa = starting ? j : g - 1;
+        const int pb = starting ? j : g;
+        double ta, tb;
+        bool na, nb;
+        const WalkKey ka = walk_key(C, T, L, mem, pa, ta, na);
+        const WalkKey kb = walk_key(C, T, L, mem, pb < last ? pb : last, tb, nb);
+        const PrefixRecord pg = mem.prefix(pb);
+        mem.count_probe();
+        bool open = false;     // a run starts at pb
+        bool end_seg = false;  // the current segment is done
+        if (starting) {
+            open = true;
+        } else {
+            const bool has1 = g - 1 > lo;
+            const bool has2 = g < seg_end;
+            const bool same1 = !has1 || walk_same(ka, k);
+            const bool same2 = has2 && walk_same(kb, k);
+            if (!same1) {                  // the run ends at g - 1 or before
+                hi = g - 1;
+                g = misses == 0 ? hi : lo + 1 + ((hi - lo) >> 1);
+                ++misses;
+            } else if (same2) {            // the run goes on past g
+                lo = g;
+                near_lo = nb;
+                g = g + step < hi ? g + step : hi;
+                step <<= 1;
+            } else {                       // the run is [j, g)
+                if ((has1 ? na : near_lo) && g - 1 > j) walk_special(C, T, L, mem, g - 1);
+                mem.count_run(pg.c - ps.c);
+                if (k.m >= -C.np) walk_flush(C, T, L, mem, k, ps, pg);
+                if (has2) {
+                    if (g - j >= 3 && tb > t_j) q = (double)(g - j) / (tb - t_j);
+                    open = true;
+                } else {
+                    end_seg = true;
+                }
+            }
+        }
+        if (open) {                        // a run starts at pixel pb with key kb
+            j = pb;
+            k = kb;
+            t_j = tb;
+            near_j = nb;
+            ps = pg;
+            starting = false;
+            if (near_j) walk_special(C, T, L, mem, j);
+            if (walk_terminal(C, k)) {
+                end_seg = true;
+            } else {
+                lo = j;
+                hi = seg_end;
+                near_lo = near_j;
+                misses = 0;
+                step = 1;
+                g = walk_guess(k, t_j, q, j, seg_end);
+            }
+        }
+        if (end_seg) {
+            if (!second && T.j_lo < split) {
+                second = true;
+                seg_end = split;
+                j = T.j_lo;
+                starting = true;
+            } else {
+                active = false;
+            }
+        }
+    }
 }
 
 }  // namespace picca

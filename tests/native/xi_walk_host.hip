@@ -214,7 +214,6 @@ int xi_walk_host(const picca_cf_params *params, int64_t num_forests, const int64
                 const double span = (r[ob + j_hi - 1] - r[ob + j_lo]) * cp;
                 if (span > 0) q0 = (double)(j_hi - 1 - j_lo) / span;
             }
-            walk = walk && q0 >= C.walk_min_q;
             if (walk) {
                 ++stats[ST_WALK_TASKS];
                 picca::WalkTask T;
