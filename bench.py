@@ -208,8 +208,8 @@ def main():
     parser.add_argument("--wick-cpu-pairs", type=int, default=3,
                         help="wick: forest pairs of the CPU-baseline sample")
     parser.add_argument("--schedule", default="auto", choices=("auto", "pairs", "runs"),
-                        help="xi: schedule of the pair kernel (auto: run walk where a bin "
-                             "holds enough pixels, else the pair loop)")
+                        help="xi: schedule of the pair kernel (auto: the pair loop, measured "
+                             "faster on these grids; runs: the run walk where it applies)")
     parser.add_argument("--spectra", type=int, default=200000,
                         help="continuum: spectra fitted (and as many more for training)")
     args = parser.parse_args()
@@ -327,6 +327,11 @@ def bench_xi(args):
             "kernel_tasks": rep["tasks"],
             "kernel_empty_tasks": rep["empty_tasks"],
             "schedule": args.schedule,
+            # what the kernel ran: the pair loop over 64-pixel chunks of delta1 (one wave
+            # per chunk and neighbour forest), the run walk, or both
+            "schedule_taken": ("pairs" if rep["walk_tasks"] == 0 else
+                               "runs" if rep["walk_tasks"] == rep["tasks"] - rep["empty_tasks"]
+                               else "runs+pairs"),
             "kernel_walk_tasks": rep["walk_tasks"],
             "kernel_runs": rep["runs"],
             "kernel_probes": rep["probes"],

@@ -134,8 +134,14 @@ int picca_xi_set_pairs(picca_ctx *ctx, int64_t num_pairs, const int32_t *pair_a,
                        const int32_t *pair_b, const double *pair_ang,
                        const uint8_t *pair_flags, const int32_t *pair_row, int32_t num_rows);
 
-/* Run the pair kernel over the resident pairs; results stay on the device. */
+/*
+ * Run the pair kernel over the resident pairs; results stay on the device.  The launch
+ * sequence (clearing kernels and the pair kernel) is captured in a HIP graph and replayed
+ * while its launch arguments stay the same; picca_xi_use_graph(ctx, 0) launches it
+ * directly instead (no counterpart in picca).
+ */
 int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_report *report);
+int picca_xi_use_graph(picca_ctx *ctx, int32_t on);
 
 /*
  * picca_xi_execute for callers that cannot allocate device memory (code under HIP graph

@@ -133,6 +133,7 @@ _PROTOTYPES = {
     "picca_xi_execute": [_P, _P, _P],
     "picca_xi_download": [_P, _P, _P, _P, _P, _P, _P],
     "picca_xi_set_schedule": [_P, _I32],
+    "picca_xi_use_graph": [_P, _I32],
     "picca_xi_execute_noalloc": [_P, _P, _P, _I64, _P],
     "picca_dmat_run": [_P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "picca_diag_copy": [_P, _I64, _P],

@@ -156,12 +156,17 @@ def compute_expected_flux(inputs, grid, num_iterations, order=1, use_constant_we
     x_keep, y_keep = np.asarray(grid, dtype=np.float64), np.ones(grid.size)
     history, kernel_ms = [], 0.0
     res = None
+    plain_start = None
     for iteration in range(num_iterations):
         kernel_ms += eng.cont_set_mean_cont(x_keep, y_keep)
-        if seeds is not None and seeds(iteration, inputs) is not None:
-            a0, b0 = seeds(iteration, inputs)
+        seed = seeds(iteration, inputs) if seeds is not None else None
+        if seed is not None:
+            a0, b0 = seed
         else:
-            a0, b0 = start_values(offsets, inputs["flux"], inputs["ivar"])
+            # picca's start values depend on flux and ivar only: the same every iteration
+            if plain_start is None:
+                plain_start = start_values(offsets, inputs["flux"], inputs["ivar"])
+            a0, b0 = plain_start
         res = eng.cont_fit(a0, b0, max_iter=max_iter, tol=tol,
                            use_constant_weight=use_constant_weight, tol_abs=tol_abs)
         kernel_ms += res["kernel_ms"]

@@ -104,6 +104,8 @@ int picca_destroy(picca_ctx *ctx) {
     if (!ctx) return PICCA_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->xi_graph_exec) (void)hipGraphExecDestroy(ctx->xi_graph_exec);
+    if (ctx->xi_graph) (void)hipGraphDestroy(ctx->xi_graph);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     if (ctx->ev_mid) (void)hipEventDestroy(ctx->ev_mid);
@@ -328,7 +330,7 @@ namespace {
 // words), all cleared here by kernels.  Allocates nothing.
 int xi_execute_into(picca_ctx *ctx, const picca::XiConst &C0, int mode,
                     const picca_cf_params *params, picca::XiBin *out, unsigned long long *queue,
-                    unsigned long long *stats_dev, picca_xi_report *report) {
+                    unsigned long long *stats_dev, bool allow_graph, picca_xi_report *report) {
     picca::XiConst C = C0;
     // the specialised kernel: standard binning, auto-correlation, r_par_min == 0, no cuts
     const bool general = !picca::xi_specialised(params, mode);
@@ -365,10 +367,6 @@ int xi_execute_into(picca_ctx *ctx, const picca::XiConst &C0, int mode,
     }
     const int64_t nrows = ctx->num_rows;
     const int64_t nout = nrows * C.nb;
-    PICCA_HIP_CHECK(picca::launch_zero_u64(reinterpret_cast<unsigned long long *>(out),
-                                           nout * (int64_t)(sizeof(picca::XiBin) / 8), ctx->stream));
-    PICCA_HIP_CHECK(picca::launch_zero_u64(queue, 1, ctx->stream));
-    PICCA_HIP_CHECK(picca::launch_zero_u64(stats_dev, picca::XI_NSTATS, ctx->stream));
 
     picca::ForestView F{ctx->pix.ptr, ctx->z.ptr, ctx->r.ptr, ctx->d.ptr, ctx->w.ptr, ctx->dw.ptr,
                         ctx->ll.ptr, ctx->off.ptr, ctx->zq.ptr, ctx->order.ptr, ctx->mono.ptr,
@@ -378,10 +376,58 @@ int xi_execute_into(picca_ctx *ctx, const picca::XiConst &C0, int mode,
     picca::GroupView G{ctx->gbegin.ptr, ctx->grow.ptr, ngroups};
     picca::XiOut O{out, queue, stats_dev};
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->num_cus, ngroups));
+    const bool xcorr = params->x_correlation != 0;
+    // the launch sequence: output, queue head and counters cleared by kernels (no memset
+    // nodes in a graph), then the pair kernel
+    auto enqueue = [&](hipStream_t st) -> hipError_t {
+        hipError_t e = picca::launch_zero_u64(reinterpret_cast<unsigned long long *>(out),
+                                              nout * (int64_t)(sizeof(picca::XiBin) / 8), st);
+        if (e == hipSuccess) e = picca::launch_zero_u64(queue, 1, st);
+        if (e == hipSuccess) e = picca::launch_zero_u64(stats_dev, picca::XI_NSTATS, st);
+        if (e == hipSuccess && ngroups > 0)
+            e = picca::launch_xi(mode, xcorr, general, lds_hist, C, F, P, G, O, grid, lds, st);
+        return e;
+    };
     PICCA_HIP_CHECK(hipEventRecord(ctx->ev_start, ctx->stream));
-    if (ngroups > 0) {
-        PICCA_HIP_CHECK(picca::launch_xi(mode, params->x_correlation != 0, general, lds_hist, C,
-                                         F, P, G, O, grid, lds, ctx->stream));
+    if (allow_graph && ctx->xi_use_graph) {
+        // one graph per set of launch arguments (everything the kernels take by value)
+        std::vector<unsigned char> key;
+        auto put = [&key](const void *v, size_t n) {
+            const unsigned char *b = (const unsigned char *)v;
+            key.insert(key.end(), b, b + n);
+        };
+        const int32_t flags[4] = {mode, xcorr ? 1 : 0, general ? 1 : 0, lds_hist ? 1 : 0};
+        put(&C, sizeof(C));
+        put(&F, sizeof(F));
+        put(&P, sizeof(P));
+        put(&G, sizeof(G));
+        put(&O, sizeof(O));
+        put(&grid, sizeof(grid));
+        put(&lds, sizeof(lds));
+        put(&nout, sizeof(nout));
+        put(flags, sizeof(flags));
+        if (!ctx->xi_graph_exec || key != ctx->xi_graph_key) {
+            if (ctx->xi_graph_exec) PICCA_HIP_CHECK(hipGraphExecDestroy(ctx->xi_graph_exec));
+            if (ctx->xi_graph) PICCA_HIP_CHECK(hipGraphDestroy(ctx->xi_graph));
+            ctx->xi_graph_exec = nullptr;
+            ctx->xi_graph = nullptr;
+            ctx->xi_graph_key.clear();
+            PICCA_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+            const hipError_t e = enqueue(ctx->stream);
+            hipGraph_t graph = nullptr;
+            const hipError_t e_end = hipStreamEndCapture(ctx->stream, &graph);
+            if (e != hipSuccess || e_end != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return fail(PICCA_E_HIP, std::string("xi graph capture: ") +
+                                             hipGetErrorString(e != hipSuccess ? e : e_end));
+            }
+            ctx->xi_graph = graph;
+            PICCA_HIP_CHECK(hipGraphInstantiate(&ctx->xi_graph_exec, graph, nullptr, nullptr, 0));
+            ctx->xi_graph_key = key;
+        }
+        PICCA_HIP_CHECK(hipGraphLaunch(ctx->xi_graph_exec, ctx->stream));
+    } else {
+        PICCA_HIP_CHECK(enqueue(ctx->stream));
     }
     PICCA_HIP_CHECK(hipEventRecord(ctx->ev_stop, ctx->stream));
     unsigned long long stats[picca::XI_NSTATS];
@@ -425,7 +471,7 @@ int picca_xi_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_rep
     PICCA_HIP_CHECK(ctx->queue.reserve(1));
     PICCA_HIP_CHECK(ctx->stats.reserve(picca::XI_NSTATS));
     status = xi_execute_into(ctx, C, mode, params, ctx->o_bins.ptr, ctx->queue.ptr,
-                             ctx->stats.ptr, report);
+                             ctx->stats.ptr, true, report);
     if (status == PICCA_OK) {
         ctx->out_nb = C.nb;
         ctx->out_rows = ctx->num_rows;
@@ -456,7 +502,13 @@ int picca_xi_execute_noalloc(picca_ctx *ctx, const picca_cf_params *params, void
     PICCA_HIP_CHECK(hipSetDevice(ctx->device));
     // scratch: word 0 the work-queue head, words 1.. the counters
     return xi_execute_into(ctx, C, mode, params, (picca::XiBin *)out_bins, ctx->scratch,
-                           ctx->scratch + 1, report);
+                           ctx->scratch + 1, false, report);
+}
+
+int picca_xi_use_graph(picca_ctx *ctx, int32_t on) {
+    if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
+    ctx->xi_use_graph = on != 0;
+    return PICCA_OK;
 }
 
 int picca_xi_set_schedule(picca_ctx *ctx, int32_t schedule) {

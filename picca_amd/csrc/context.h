@@ -125,6 +125,12 @@ struct picca_ctx {
     DeviceBuffer<double2> rd, spc;
     DeviceBuffer<picca::PrefixRecord> pre;
     int32_t xi_schedule = 0;   // PICCA_XI_SCHEDULE_*
+    // the xi launch sequence (zero kernels + pair kernel) as a HIP graph, replayed while
+    // its launch arguments stay the same
+    bool xi_use_graph = true;
+    hipGraph_t xi_graph = nullptr;
+    hipGraphExec_t xi_graph_exec = nullptr;
+    std::vector<unsigned char> xi_graph_key;
 
     // xi pairs
     int64_t num_pairs = 0;

@@ -155,6 +155,10 @@ class Engine:
         (we, xi, rp, rt, z), nb = self.xi_download(params)
         return XiResult(we, xi, rp, rt, z, nb, report.as_dict())
 
+    def use_xi_graph(self, on):
+        """Launch xi through its captured HIP graph (default) or directly."""
+        check(self._lib.picca_xi_use_graph(self._ctx, int(bool(on))), "picca_xi_use_graph")
+
     def set_xi_schedule(self, schedule):
         """Schedule of the specialised pair kernel: _native.XI_SCHEDULE_AUTO, _PAIRS or
         _RUNS (picca_xi_set_schedule)."""
