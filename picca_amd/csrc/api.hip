@@ -928,8 +928,11 @@ int picca_dmat_run(picca_ctx *ctx, const picca_cf_params *params, int64_t num_pa
     const size_t per_row = 32 * (size_t)C.np + 64 * (size_t)C.npm;
     if (fixed + per_row > lds_budget)
         return fail(PICCA_E_VALUE, "too many bins for the distortion-matrix pass kernel");
-    const int k_rows = (int)std::min<size_t>((lds_budget - fixed) / per_row,
-                                             (size_t)std::max(C.nt, C.ntm));
+    // at most DMAT_PASS_ROWS rows per band: a pair's rows usually fit (one walk does census
+    // and sums), and the smaller LDS lets several pair workgroups share a CU
+    const int k_rows = (int)std::min<size_t>(
+        {(lds_budget - fixed) / per_row, (size_t)std::max(C.nt, C.ntm),
+         (size_t)picca::DMAT_PASS_ROWS});
     C.tb_max = std::min(C.nt, k_rows);
     C.ub_max = std::min(C.ntm, k_rows);
     const size_t pass_lds = fixed + 32 * (size_t)C.np * C.tb_max + 64 * (size_t)C.npm * C.ub_max;

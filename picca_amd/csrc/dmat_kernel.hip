@@ -21,10 +21,14 @@
 //                     count-pass truncation), and for each r_trans row the range of
 //                     pixels that reach it;
 //   dmat_fill_kernel  one workgroup per (data r_trans row t, model r_trans row u, slot
-//                     group): an np x npm block of dmat held in registers, accumulating
-//                     the S x eta rank-4 term, and per pixel i (then j) the vectors P, P',
-//                     eta1, eta3 (Q, Q', eta2, eta4) built by walking that pixel's pairs,
-//                     followed by their outer products; one atomic add per block entry.
+//                     group): an np x npm block of dmat held in the matrix cores'
+//                     accumulators, adding the S x eta rank-4 term, and per four pixels i
+//                     (then j) the vectors P, P', eta1, eta3 (Q, Q', eta2, eta4) built by
+//                     walking those pixels' pairs, followed by P^T eta1 + P'^T eta3 as two
+//                     K = 4 FP64 products (v_mfma_f64_16x16x4f64, K = the pixels); one
+//                     atomic add per block entry.
+// The pair pass walks every pixel pair once when the pair's r_trans rows fit its LDS bands
+// (census and sums together), else twice (census, then the sums band by band).
 // All arithmetic is FP64; bins follow the reference's divide-then-multiply.
 //
 // The forest x quasar distortion matrix (compute_dmat_forest_pairs_fast of
@@ -205,6 +209,94 @@ __device__ void row_window(const DmatConst &C, const DmatPair &q, const double *
     hi = min(hi, h);
 }
 
+// The searches of window() and row_window() for one pixel, done together: thresholds 0, 1 on
+// the partner's r_comov (first index >= x0, first index > x1), 2..4 on its dist_m (first
+// > x2, first >= x3, first > x4; only the first `k` thresholds are searched).  One round of
+// 64 probes serves every threshold, then one probe round per threshold inside its
+// sub-interval, all issued together: two dependent memory rounds instead of two per
+// threshold.  Forests longer than 4096 pixels take wave_search.
+__device__ void search_rd(const double *__restrict__ R, const double *__restrict__ D, int n,
+                          const double (&x)[5], int k, int lane, int (&res)[5]) {
+    if (n > 4096) {
+        res[0] = wave_search<true>(R, n, x[0], lane);
+        res[1] = wave_search<false>(R, n, x[1], lane);
+        if (k > 2) {
+            res[2] = wave_search<false>(D, n, x[2], lane);
+            res[3] = wave_search<true>(D, n, x[3], lane);
+            res[4] = wave_search<false>(D, n, x[4], lane);
+        }
+        return;
+    }
+    const int stride = (n + 63) >> 6;   // <= 64
+    const int probe = lane * stride;
+    const bool ok = probe < n;
+    double vr = 0.0, vd = 0.0;
+    if (ok) {
+        vr = R[probe];
+        if (k > 2) vd = D[probe];
+    }
+    int sub_lo[5], sub_len[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        sub_lo[q] = 0;
+        sub_len[q] = 0;
+        if (q >= k) continue;
+        const double v = q < 2 ? vr : vd;
+        const bool below = ok && ((q == 0 || q == 3) ? v < x[q] : v <= x[q]);
+        const int cnt = __popcll(__ballot(below));
+        if (cnt > 0) {
+            const int start = (cnt - 1) * stride + 1;
+            const int stop = min(cnt * stride, n);
+            sub_lo[q] = start;
+            sub_len[q] = stop - start;   // < 64
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        if (q >= k) continue;
+        bool below = false;
+        if (lane < sub_len[q]) {
+            const double v = (q < 2 ? R : D)[sub_lo[q] + lane];
+            below = (q == 0 || q == 3) ? v < x[q] : v <= x[q];
+        }
+        res[q] = uniform_int(sub_lo[q] + __popcll(__ballot(below)));
+    }
+}
+
+// window_b / window_a followed by row_window for one pixel (r_comov r_self, dist_m d_self)
+// of the other forest (Or, Od, n): the same [lo, hi), from search_rd.  `r_lo`, `r_hi` are
+// the partner r_comov bounds of window_b / window_a before their slack.
+__device__ void pixel_window(const DmatConst &C, const DmatPair &q, const double *__restrict__ Or,
+                             const double *__restrict__ Od, int n, double r_lo, double r_hi,
+                             double d_self, int t, int u, int lane, int &lo, int &hi) {
+    lo = 0;
+    hi = n;
+    if (!q.prune) return;
+    double x[5];
+    x[0] = r_lo - (1e-6 + 1e-12 * fabs(r_lo));
+    x[1] = r_hi + (1e-6 + 1e-12 * fabs(r_hi));
+    const bool use_d = q.sin_half > 0;
+    x[2] = x[3] = x[4] = 0.0;
+    if (use_d) {
+        const double d_lim = C.rtmax / q.sin_half - d_self;
+        x[2] = d_lim + (1e-6 + 1e-12 * fabs(d_lim));
+        const double r_row_lo = fmin(t * C.rtmax / C.nt_d, u * C.rtmax / C.ntm_d);
+        const double r_row_hi = fmax((t + 1) * C.rtmax / C.nt_d, (u + 1) * C.rtmax / C.ntm_d);
+        const double slack = 1e-6 + 1e-9 * (fabs(r_row_hi / q.sin_half) + fabs(d_self));
+        x[3] = r_row_lo / q.sin_half - d_self - slack;
+        x[4] = r_row_hi / q.sin_half - d_self + slack;
+    }
+    int res[5];
+    search_rd(Or, Od, n, x, use_d ? 5 : 2, lane, res);
+    lo = res[0];
+    hi = res[1];
+    if (use_d) {
+        hi = min(hi, res[2]);
+        lo = max(lo, res[3]);
+        hi = min(hi, res[4]);
+    }
+}
+
 __device__ __forceinline__ void lds_add(double *p, double v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
@@ -292,8 +384,157 @@ __global__ __launch_bounds__(DMAT_PASS_BLOCK) void dmat_pass_kernel(DmatConst C,
     const bool evol = (C.flags & DMAT_REDSHIFT_EVOL) != 0;
     const int64_t pix = W.pix_off[s];
 
+    // run flushes into the LDS bands that start at data row tc0 / model row uc0
+    auto flush_a = [&](const RunA &ra, int tc0) {
+        const int t = ra.key % C.nt;
+        if (t < tc0 || t >= tc0 + C.tb_max) return;
+        double *b = lS + (t - tc0) * C.np + ra.key / C.nt;
+        const int stride = C.tb_max * C.np;
+        lds_add(b, ra.s0);
+        lds_add(b + stride, ra.sj);
+        lds_add(b + 2 * stride, ra.si);
+        lds_add(b + 3 * stride, ra.sij);
+    };
+    auto flush_m = [&](const RunM &rm, int uc0) {
+        const int u = rm.key % C.ntm;
+        if (u < uc0 || u >= uc0 + C.ub_max) return;
+        double *b = lE + (u - uc0) * C.npm + rm.key / C.ntm;
+        const int stride = C.ub_max * C.npm;
+        lds_add(b, rm.e5);
+        lds_add(b + stride, rm.e6);
+        lds_add(b + 2 * stride, rm.e7);
+        lds_add(b + 3 * stride, rm.e8);
+        lds_add(b + 4 * stride, rm.rp);
+        lds_add(b + 5 * stride, rm.rt);
+        lds_add(b + 6 * stride, rm.z);
+        lds_add(b + 7 * stride, rm.w);
+    };
+
+    // ---- one walk for census and sums when the pair's rows fit the LDS bands ------------
+    // Without the same-half-plate truncation of U or the count pass of xcf, the census and
+    // the accumulation can share one walk if every r_trans row the pair can reach is in the
+    // bands from the start: r_trans = (d1 + d2) sin(ang/2) is non-decreasing in both
+    // (monotone) dist_m, and pair_bins' row arithmetic is non-decreasing in r_trans, so the
+    // rows of the first and last pixels' sums bound every pair's rows.
+    int g_t0 = 0, g_u0 = 0;
+    bool merged = false;
+    if (!q.shp_cut && !q.strict && q.prune && q.la > 0 && q.lb > 0) {
+        const double y_lo = (Ad[0] + Bd[0]) * q.sin_half / C.rtmax;
+        const double y_hi = (Ad[q.la - 1] + Bd[q.lb - 1]) * q.sin_half / C.rtmax;
+        const double t0 = fmax(floor(y_lo * C.nt_d), 0.0);
+        const double t1 = fmin(floor(y_hi * C.nt_d), C.nt_d - 1);
+        const double u0 = fmax(floor(y_lo * C.ntm_d), 0.0);
+        const double u1 = fmin(floor(y_hi * C.ntm_d), C.ntm_d - 1);
+        merged = t0 <= t1 && u0 <= u1 && t1 - t0 < C.tb_max && u1 - u0 < C.ub_max;
+        if (merged) {
+            g_t0 = (int)t0;
+            g_u0 = (int)u0;
+        }
+    }
+    if (merged) {
+        for (int k = tid; k < 4 * C.tb_max * C.np; k += DMAT_PASS_BLOCK) lS[k] = 0;
+        for (int k = tid; k < 8 * C.ub_max * C.npm; k += DMAT_PASS_BLOCK) lE[k] = 0;
+        __syncthreads();
+        uint64_t n_in = 0, n_sel = 0, n_over = 0, n_neg = 0;
+        for (int c0 = wave * 64; c0 < q.la; c0 += NWAVES * 64) {
+            const int i = c0 + lane;
+            bool active = i < q.la;
+            double r1 = 0, d1 = 0, z1 = 0, w1 = 0, l1 = 0, f1 = 1;
+            if (active) {
+                r1 = Ar[i];
+                d1 = Ad[i];
+                z1 = Az[i];
+                w1 = Aw[i];
+                l1 = Al[i];
+                if (evol) f1 = Af[i];
+                active = w1 != 0;
+            }
+            const bool i_sel = !(q.zerr_on && zerr_cut(C, z1, q.zq2));
+            const double a7 = q.order1 ? w1 * l1 * q.inv_s1 : 0.0;
+            int j_lo, j_hi;
+            window_b(C, q, F, wave_min(active ? r1 : INFINITY),
+                     wave_max(active ? r1 : -INFINITY), wave_min(active ? d1 : INFINITY), lane,
+                     j_lo, j_hi);
+            RowRun rt{-1, 0, 0}, ru{-1, 0, 0};
+            int last_m = -1;
+            RunA ra{-1, 0, 0, 0, 0};
+            RunM rm{-1, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int j = j_lo; j < j_hi; ++j) {
+                const double w2 = Bw[j];
+                if (w2 == 0) continue;
+                const double r2 = Br[j], d2 = Bd[j], z2 = Bz[j], l2 = Bl[j];
+                const double f2 = evol ? Bf[j] : 1.0;
+                const bool j_zerr = q.zerr_on && zerr_cut(C, z2, q.zq1);
+                double r_par, r_trans;
+                int A = 0, M = 0;
+                if (!active || !pair_bins(C, r1, d1, r2, d2, q.cos_half, q.sin_half, r_par,
+                                          r_trans, A, M))
+                    continue;
+                ++n_in;   // the count pass takes every in-range pair here
+                if (M < 0 || M >= C.nm) continue;
+                const double z = (z1 + z2) / 2;
+                if (M != last_m) {
+                    atomicOr(&lU[M >> 5], 1u << (M & 31));
+                    last_m = M;
+                }
+                row_step(ru, M % C.ntm, j, l_iu, l_ju, i);
+                const double w12 = w1 * w2;
+                const double zfac = evol ? f1 * f2 : 1.0;
+                const bool sel = pair_selected(C, q, i_sel, j_zerr, z, r_par);
+                if (M != rm.key) {
+                    if (rm.key >= 0) flush_m(rm, g_u0);
+                    rm = RunM{M, 0, 0, 0, 0, 0, 0, 0, 0};
+                }
+                if (q.proj2) rm.e5 += zfac * w12 * q.inv_w1 * q.inv_w2;
+                if (q.order2) rm.e6 += zfac * w1 * q.inv_w1 * (w2 * l2 * q.inv_s2);
+                if (q.order1) {
+                    if (q.proj2) rm.e7 += zfac * w2 * q.inv_w2 * a7;
+                    if (q.order2) rm.e8 += zfac * w1 * l1 * w2 * l2 * q.inv_s1 * q.inv_s2;
+                }
+                if (!sel) continue;
+                ++n_sel;
+                rm.rp += w12 * r_par;
+                rm.rt += w12 * r_trans;
+                rm.z += w12 * z;
+                rm.w += w12;
+                if (A < 0) {
+                    ++n_neg;
+                    continue;
+                }
+                if (A >= C.nb) {
+                    ++n_over;
+                    continue;
+                }
+                row_step(rt, A % C.nt, j, l_it, l_jt, i);
+                if (A != ra.key) {
+                    if (ra.key >= 0) flush_a(ra, g_t0);
+                    ra = RunA{A, 0, 0, 0, 0};
+                }
+                ra.s0 += w12;
+                ra.sj += w12 * l2;
+                ra.si += w12 * l1;
+                ra.sij += w12 * l1 * l2;
+            }
+            if (rt.row >= 0) note_run(l_it, l_jt, rt.row, i, rt.first, rt.last);
+            if (ru.row >= 0) note_run(l_iu, l_ju, ru.row, i, ru.first, ru.last);
+            if (ra.key >= 0) flush_a(ra, g_t0);
+            if (rm.key >= 0) flush_m(rm, g_u0);
+        }
+        n_in = wave_sum_u64(n_in);
+        n_sel = wave_sum_u64(n_sel);
+        n_over = wave_sum_u64(n_over);
+        n_neg = wave_sum_u64(n_neg);
+        if (lane == 0) {
+            atomicAdd(&s_cnt[0], (unsigned long long)n_in);
+            atomicAdd(&s_cnt[1], (unsigned long long)n_sel);
+            atomicAdd(&s_cnt[2], (unsigned long long)n_in);
+            atomicAdd(&s_cnt[3], (unsigned long long)n_over);
+            atomicAdd(&s_cnt[4], (unsigned long long)n_neg);
+        }
+    }
+
     // ---- walk 0: census (counts, pixel ranges of every row, U unless truncated) ---------
-    {
+    if (!merged) {
         uint64_t n_in = 0, n_sel = 0, n_cp = 0, n_over = 0, n_neg = 0;
         for (int c0 = wave * 64; c0 < q.la; c0 += NWAVES * 64) {
             const int i = c0 + lane;
@@ -482,37 +723,17 @@ __global__ __launch_bounds__(DMAT_PASS_BLOCK) void dmat_pass_kernel(DmatConst C,
     const int t_lo = s_band[0], t_hi = s_band[1], u_lo = s_band[2], u_hi = s_band[3];
     const int n_t = t_hi >= t_lo ? t_hi - t_lo + 1 : 0;
     const int n_u = u_hi >= u_lo ? u_hi - u_lo + 1 : 0;
-    const int passes = max((n_t + C.tb_max - 1) / C.tb_max, (n_u + C.ub_max - 1) / C.ub_max);
+    // (the single walk above has filled the one band pass already)
+    const int passes = merged ? (n_t > 0 || n_u > 0 ? 1 : 0)
+                              : max((n_t + C.tb_max - 1) / C.tb_max,
+                                    (n_u + C.ub_max - 1) / C.ub_max);
     for (int pass = 0; pass < passes; ++pass) {
-        const int tc0 = t_lo + pass * C.tb_max;
-        const int uc0 = u_lo + pass * C.ub_max;
+        const int tc0 = merged ? g_t0 : t_lo + pass * C.tb_max;
+        const int uc0 = merged ? g_u0 : u_lo + pass * C.ub_max;
+        if (!merged) {
         for (int k = tid; k < 4 * C.tb_max * C.np; k += DMAT_PASS_BLOCK) lS[k] = 0;
         for (int k = tid; k < 8 * C.ub_max * C.npm; k += DMAT_PASS_BLOCK) lE[k] = 0;
         __syncthreads();
-        auto flush_a = [&](const RunA &ra) {
-            const int t = ra.key % C.nt;
-            if (t < tc0 || t >= tc0 + C.tb_max) return;
-            double *b = lS + (t - tc0) * C.np + ra.key / C.nt;
-            const int stride = C.tb_max * C.np;
-            lds_add(b, ra.s0);
-            lds_add(b + stride, ra.sj);
-            lds_add(b + 2 * stride, ra.si);
-            lds_add(b + 3 * stride, ra.sij);
-        };
-        auto flush_m = [&](const RunM &rm) {
-            const int u = rm.key % C.ntm;
-            if (u < uc0 || u >= uc0 + C.ub_max) return;
-            double *b = lE + (u - uc0) * C.npm + rm.key / C.ntm;
-            const int stride = C.ub_max * C.npm;
-            lds_add(b, rm.e5);
-            lds_add(b + stride, rm.e6);
-            lds_add(b + 2 * stride, rm.e7);
-            lds_add(b + 3 * stride, rm.e8);
-            lds_add(b + 4 * stride, rm.rp);
-            lds_add(b + 5 * stride, rm.rt);
-            lds_add(b + 6 * stride, rm.z);
-            lds_add(b + 7 * stride, rm.w);
-        };
         for (int c0 = wave * 64; c0 < q.la; c0 += NWAVES * 64) {
             const int i = c0 + lane;
             bool active = i < q.la;
@@ -552,7 +773,7 @@ __global__ __launch_bounds__(DMAT_PASS_BLOCK) void dmat_pass_kernel(DmatConst C,
                 const double zfac = evol ? f1 * f2 : 1.0;
                 const bool sel = pair_selected(C, q, i_sel, j_zerr, z, r_par);
                 if (M != rm.key) {
-                    if (rm.key >= 0) flush_m(rm);
+                    if (rm.key >= 0) flush_m(rm, uc0);
                     rm = RunM{M, 0, 0, 0, 0, 0, 0, 0, 0};
                 }
                 if (q.proj2) rm.e5 += zfac * w12 * q.inv_w1 * q.inv_w2;
@@ -568,7 +789,7 @@ __global__ __launch_bounds__(DMAT_PASS_BLOCK) void dmat_pass_kernel(DmatConst C,
                 rm.w += w12;
                 if (A < 0 || A >= C.nb) continue;
                 if (A != ra.key) {
-                    if (ra.key >= 0) flush_a(ra);
+                    if (ra.key >= 0) flush_a(ra, tc0);
                     ra = RunA{A, 0, 0, 0, 0};
                 }
                 ra.s0 += w12;
@@ -576,10 +797,11 @@ __global__ __launch_bounds__(DMAT_PASS_BLOCK) void dmat_pass_kernel(DmatConst C,
                 ra.si += w12 * l1;
                 ra.sij += w12 * l1 * l2;
             }
-            if (ra.key >= 0) flush_a(ra);
-            if (rm.key >= 0) flush_m(rm);
+            if (ra.key >= 0) flush_a(ra, tc0);
+            if (rm.key >= 0) flush_m(rm, uc0);
         }
         __syncthreads();
+        }
         // band rows -> workspace (eta masked by U), weights_dmat and effective sums -> out
         const int t_end = min(t_hi + 1, tc0 + C.tb_max);
         for (int k = tid; k < (t_end - tc0) * C.np; k += DMAT_PASS_BLOCK) {
@@ -622,6 +844,11 @@ constexpr int VEC_E = 2 * DMAT_CHUNK_RP;                  // eta1 (or eta2) [CHU
 constexpr int VEC_E2 = 2 * DMAT_CHUNK_RP + DMAT_CHUNK_MRP;  // eta3 (or eta4)
 constexpr int VEC_LEN = 2 * DMAT_CHUNK_RP + 2 * DMAT_CHUNK_MRP;
 constexpr int FILL_WAVES = DMAT_FILL_BLOCK / 64;
+constexpr int FILL_CB = DMAT_CHUNK_MRP / 16;   // 16-column blocks of model r_par bins
+static_assert(DMAT_CHUNK_RP == 16 * FILL_WAVES, "one 16-row block of data bins per wave");
+static_assert(DMAT_CHUNK_MRP % 16 == 0, "model bins in 16-column blocks");
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
 
 constexpr size_t FILL_LDS_DOUBLES = (size_t)DMAT_CHUNK_RP * DMAT_CHUNK_MRP   // diag block
                                     + 4 * DMAT_CHUNK_RP + 4 * DMAT_CHUNK_MRP  // S, eta rows
@@ -645,8 +872,12 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int tx = tid & 15;   // owner of model r_par bins tx + 16 b
-    const int ty = tid >> 4;   // owner of data r_par bins ty + 16 a
+    // v_mfma_f64_16x16x4f64 operands: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16];
+    // register r of the result holds row 4 r + l / 16, column l % 16 of the 16 x 16 block.
+    // Wave w owns the data r_par bins 16 w .. 16 w + 15 of the chunk and every model bin.
+    const int li = lane & 15;
+    const int lk = lane >> 4;
+    const int row = 16 * wave + li;   // this lane's data bin in the A operands
     const bool evol = (C.flags & DMAT_REDSHIFT_EVOL) != 0;
 
     double *diag = (double *)smem;                              // [CHUNK_RP][CHUNK_MRP]
@@ -660,34 +891,29 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
         for (int mrp0 = 0; mrp0 < C.npm; mrp0 += DMAT_CHUNK_MRP) {
             const int nrp = min(DMAT_CHUNK_RP, C.np - rp0);
             const int nmrp = min(DMAT_CHUNK_MRP, C.npm - mrp0);
-            double acc[DMAT_FILL_RP][DMAT_FILL_MRP];
+            double4_t acc[FILL_CB];
 #pragma unroll
-            for (int a = 0; a < DMAT_FILL_RP; ++a)
-#pragma unroll
-                for (int b = 0; b < DMAT_FILL_MRP; ++b) acc[a][b] = 0;
+            for (int b = 0; b < FILL_CB; ++b) acc[b] = (double4_t){0.0, 0.0, 0.0, 0.0};
             for (int k = tid; k < DMAT_CHUNK_RP * DMAT_CHUNK_MRP; k += DMAT_FILL_BLOCK)
                 diag[k] = 0;
             for (int k = tid; k < FILL_WAVES * VEC_LEN; k += DMAT_FILL_BLOCK) vecs[k] = 0;
             __syncthreads();
 
             auto outer = [&](int nvec) {
-                // acc -= sum over the first nvec waves' vectors of P x E + P2 x E2
-                for (int w = 0; w < nvec; ++w) {
-                    const double *v = vecs + w * VEC_LEN;
-                    double p[DMAT_FILL_RP], p2[DMAT_FILL_RP];
+                // acc -= sum over the first nvec waves' vectors of P x E + P2 x E2: two
+                // K = 4 products on the matrix cores (K = the pixels), A = -P, B = E
+                const bool kin = lk < nvec;
+                const double *v = vecs + lk * VEC_LEN;
+                const double a1 = kin ? -v[VEC_P + row] : 0.0;
+                const double a2 = kin ? -v[VEC_P2 + row] : 0.0;
+                // no pair of these pixels in this wave's data bins: nothing to add
+                if (__ballot(a1 != 0.0 || a2 != 0.0) == 0) return;
 #pragma unroll
-                    for (int a = 0; a < DMAT_FILL_RP; ++a) {
-                        p[a] = v[VEC_P + ty + 16 * a];
-                        p2[a] = v[VEC_P2 + ty + 16 * a];
-                    }
-#pragma unroll
-                    for (int b = 0; b < DMAT_FILL_MRP; ++b) {
-                        const double e = v[VEC_E + tx + 16 * b];
-                        const double e2 = v[VEC_E2 + tx + 16 * b];
-#pragma unroll
-                        for (int a = 0; a < DMAT_FILL_RP; ++a)
-                            acc[a][b] -= p[a] * e + p2[a] * e2;
-                    }
+                for (int b = 0; b < FILL_CB; ++b) {
+                    const double e1 = kin ? v[VEC_E + 16 * b + li] : 0.0;
+                    const double e2 = kin ? v[VEC_E2 + 16 * b + li] : 0.0;
+                    acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, e1, acc[b], 0, 0, 0);
+                    acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, e2, acc[b], 0, 0, 0);
                 }
             };
 
@@ -723,17 +949,14 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
                     mask[m] = v;
                 }
                 __syncthreads();
+                if (!QUASAR) {
+                    // rank-4 term: one K = 4 product per block (K = the four sums)
+                    const double sa = vS[lk * DMAT_CHUNK_RP + row];
+                    if (__ballot(sa != 0.0) != 0) {
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    if (QUASAR) break;
-                    double sv[DMAT_FILL_RP];
-#pragma unroll
-                    for (int a = 0; a < DMAT_FILL_RP; ++a) sv[a] = vS[c * DMAT_CHUNK_RP + ty + 16 * a];
-#pragma unroll
-                    for (int b = 0; b < DMAT_FILL_MRP; ++b) {
-                        const double e = vE[c * DMAT_CHUNK_MRP + tx + 16 * b];
-#pragma unroll
-                        for (int a = 0; a < DMAT_FILL_RP; ++a) acc[a][b] += sv[a] * e;
+                        for (int b = 0; b < FILL_CB; ++b)
+                            acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                sa, vE[lk * DMAT_CHUNK_MRP + 16 * b + li], acc[b], 0, 0, 0);
                     }
                 }
 
@@ -767,8 +990,11 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
                                 const double f1 = evol ? Af[i] : 1.0;
                                 const bool i_sel = !(q.zerr_on && zerr_cut(C, z1, q.zq2));
                                 int j_lo, j_hi;
-                                window_b(C, q, F, r1, r1, d1, lane, j_lo, j_hi);
-                                row_window(C, q, Bd, q.lb, d1, t, u, lane, j_lo, j_hi);
+                                pixel_window(C, q, Br, Bd, q.lb, r1 - C.rpmax / q.cos_half,
+                                             (C.flags & DMAT_XCORR)
+                                                 ? r1 - C.rpmin / q.cos_half
+                                                 : r1 + C.rpmax / q.cos_half,
+                                             d1, t, u, lane, j_lo, j_hi);
                                 for (int j = j_lo + lane; j < j_hi; j += 64) {
                                     const double w2 = Bw[j];
                                     if (w2 == 0) continue;
@@ -879,8 +1105,12 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
                                 const double f2 = evol ? Bf[j] : 1.0;
                                 const bool j_zerr = q.zerr_on && zerr_cut(C, z2, q.zq1);
                                 int i_lo, i_hi;
-                                window_a(C, q, F, r2, d2, lane, i_lo, i_hi);
-                                row_window(C, q, Ad, q.la, d2, t, u, lane, i_lo, i_hi);
+                                pixel_window(C, q, Ar, Ad, q.la,
+                                             (C.flags & DMAT_XCORR)
+                                                 ? r2 + C.rpmin / q.cos_half
+                                                 : r2 - C.rpmax / q.cos_half,
+                                             r2 + C.rpmax / q.cos_half, d2, t, u, lane, i_lo,
+                                             i_hi);
                                 for (int i = i_lo + lane; i < i_hi; i += 64) {
                                     const double w1 = Aw[i];
                                     if (w1 == 0) continue;
@@ -922,13 +1152,13 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
 
             // block (+ the pair-bin term) -> dmat
 #pragma unroll
-            for (int a = 0; a < DMAT_FILL_RP; ++a) {
-                const int r = ty + 16 * a;
+            for (int a = 0; a < 4; ++a) {
+                const int r = 16 * wave + 4 * a + lk;
 #pragma unroll
-                for (int b = 0; b < DMAT_FILL_MRP; ++b) {
-                    const int m = tx + 16 * b;
+                for (int b = 0; b < FILL_CB; ++b) {
+                    const int m = 16 * b + li;
                     if (r < nrp && m < nmrp) {
-                        const double v = acc[a][b] + diag[r * DMAT_CHUNK_MRP + m];
+                        const double v = acc[b][a] + diag[r * DMAT_CHUNK_MRP + m];
                         if (v != 0) {
                             const int64_t A = (int64_t)(rp0 + r) * C.nt + t;
                             const int64_t k = (int64_t)(mrp0 + m) * C.ntm + u;

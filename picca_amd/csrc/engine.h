@@ -132,9 +132,10 @@ constexpr size_t XI_TILE_PIXEL_BYTES = 64;
 // ---------------------------------------------------------------------------------------
 
 constexpr int DMAT_PASS_BLOCK = 256;   // threads of the pair-pass kernel (lane per pixel i)
-constexpr int DMAT_FILL_BLOCK = 256;   // threads of the fill kernel (16 x 16 block owners)
-constexpr int DMAT_FILL_RP = 4;        // data r_par bins per owner thread  (64 per chunk)
-constexpr int DMAT_FILL_MRP = 7;       // model r_par bins per owner thread (112 per chunk)
+constexpr int DMAT_PASS_ROWS = 6;      // r_trans rows per LDS band of the pair pass
+constexpr int DMAT_FILL_BLOCK = 256;   // threads of the fill kernel (4 waves)
+constexpr int DMAT_FILL_RP = 4;        // 16-bin blocks of data r_par bins per chunk (one per wave)
+constexpr int DMAT_FILL_MRP = 7;       // 16-bin blocks of model r_par bins per chunk
 constexpr int DMAT_CHUNK_RP = 16 * DMAT_FILL_RP;
 constexpr int DMAT_CHUNK_MRP = 16 * DMAT_FILL_MRP;
 
