@@ -850,10 +850,21 @@ static_assert(DMAT_CHUNK_MRP % 16 == 0, "model bins in 16-column blocks");
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-constexpr size_t FILL_LDS_DOUBLES = (size_t)DMAT_CHUNK_RP * DMAT_CHUNK_MRP   // diag block
-                                    + 4 * DMAT_CHUNK_RP + 4 * DMAT_CHUNK_MRP  // S, eta rows
-                                    + DMAT_CHUNK_MRP                          // U mask
-                                    + (size_t)FILL_WAVES * VEC_LEN;          // vectors
+constexpr size_t FILL_LDS_FIXED = 4 * DMAT_CHUNK_RP + 4 * DMAT_CHUNK_MRP  // S, eta rows
+                                  + DMAT_CHUNK_MRP                          // U mask
+                                  + (size_t)FILL_WAVES * VEC_LEN;          // vectors
+
+// The pair-bin term w1 w2 zfac of a pair goes to its data bin A and its model bin M, which
+// share x = (r_par - r_par_min) / span: the model r_par bin of data r_par bin a lies in
+// [a npm / np, (a + 1) npm / np], so the fill keeps, per data bin, `diag_width` model bins
+// from diag_lo(a) (one bin of slack each side) instead of the whole block; a term outside
+// them (rounding beyond the slack: not expected) goes to dmat directly.
+__host__ __device__ inline int diag_width(const DmatConst &C) {
+    return min((C.npm + C.np - 1) / C.np + 3, DMAT_CHUNK_MRP);
+}
+__device__ __forceinline__ int diag_lo(const DmatConst &C, int rp) {
+    return (int)(((int64_t)rp * C.npm) / C.np) - 1;
+}
 
 }  // namespace
 
@@ -861,9 +872,9 @@ constexpr size_t FILL_LDS_DOUBLES = (size_t)DMAT_CHUNK_RP * DMAT_CHUNK_MRP   // 
 // no eta1, eta3, eta5..eta8 terms; a separate instantiation keeps the forest x forest form's
 // register budget.
 template <bool QUASAR>
-__global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C, DmatForests F,
-                                                                    DmatPairs P, DmatWork W,
-                                                                    DmatOut O, int groups) {
+__global__ __launch_bounds__(DMAT_FILL_BLOCK, 3) void dmat_fill_kernel(DmatConst C, DmatForests F,
+                                                                       DmatPairs P, DmatWork W,
+                                                                       DmatOut O, int groups) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int tile = blockIdx.x;
     if (!((W.touched[tile >> 5] >> (tile & 31)) & 1u)) return;
@@ -880,8 +891,9 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
     const int row = 16 * wave + li;   // this lane's data bin in the A operands
     const bool evol = (C.flags & DMAT_REDSHIFT_EVOL) != 0;
 
-    double *diag = (double *)smem;                              // [CHUNK_RP][CHUNK_MRP]
-    double *vS = diag + DMAT_CHUNK_RP * DMAT_CHUNK_MRP;         // [4][CHUNK_RP]
+    const int nd = diag_width(C);
+    double *diag = (double *)smem;                              // [CHUNK_RP][nd]
+    double *vS = diag + DMAT_CHUNK_RP * nd;                     // [4][CHUNK_RP]
     double *vE = vS + 4 * DMAT_CHUNK_RP;                        // [4][CHUNK_MRP]
     double *mask = vE + 4 * DMAT_CHUNK_MRP;                     // [CHUNK_MRP]
     double *vecs = mask + DMAT_CHUNK_MRP;                       // [FILL_WAVES][VEC_LEN]
@@ -894,8 +906,18 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
             double4_t acc[FILL_CB];
 #pragma unroll
             for (int b = 0; b < FILL_CB; ++b) acc[b] = (double4_t){0.0, 0.0, 0.0, 0.0};
-            for (int k = tid; k < DMAT_CHUNK_RP * DMAT_CHUNK_MRP; k += DMAT_FILL_BLOCK)
-                diag[k] = 0;
+            for (int k = tid; k < DMAT_CHUNK_RP * nd; k += DMAT_FILL_BLOCK) diag[k] = 0;
+            // pair-bin term of data bin r and model bin m of the chunk
+            auto add_diag = [&](int r, int m, double v) {
+                const int k = mrp0 + m - diag_lo(C, rp0 + r);
+                if (k >= 0 && k < nd) {
+                    lds_add(diag + r * nd + k, v);
+                } else {
+                    const int64_t A = (int64_t)(rp0 + r) * C.nt + t;
+                    const int64_t M = (int64_t)(mrp0 + m) * C.ntm + u;
+                    atomicAdd(&O.dmat[A * C.nm + M], v);
+                }
+            };
             for (int k = tid; k < FILL_WAVES * VEC_LEN; k += DMAT_FILL_BLOCK) vecs[k] = 0;
             __syncthreads();
 
@@ -1023,7 +1045,7 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
                                     const double w12 = w1 * w2;
                                     lds_add(vec + VEC_P + r, w12);
                                     lds_add(vec + VEC_P2 + r, w12 * l2);
-                                    if (in_m) lds_add(diag + r * DMAT_CHUNK_MRP + m, w12 * zfac);
+                                    if (in_m) add_diag(r, m, w12 * zfac);
                                 }
                             }
                         }
@@ -1081,7 +1103,7 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
                             const double w12 = w1 * w2;
                             lds_add(sv + VEC_P + r, w12);
                             lds_add(sv + VEC_P2 + r, w12 * l1);
-                            if (in_m) lds_add(diag + r * DMAT_CHUNK_MRP + m, w12 * zfac);
+                            if (in_m) add_diag(r, m, w12 * zfac);
                         }
                         __syncthreads();
                         outer(1);
@@ -1158,7 +1180,9 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
                 for (int b = 0; b < FILL_CB; ++b) {
                     const int m = 16 * b + li;
                     if (r < nrp && m < nmrp) {
-                        const double v = acc[b][a] + diag[r * DMAT_CHUNK_MRP + m];
+                        const int k = mrp0 + m - diag_lo(C, rp0 + r);
+                        const double v =
+                            acc[b][a] + ((k >= 0 && k < nd) ? diag[r * nd + k] : 0.0);
                         if (v != 0) {
                             const int64_t A = (int64_t)(rp0 + r) * C.nt + t;
                             const int64_t k = (int64_t)(mrp0 + m) * C.ntm + u;
@@ -1172,7 +1196,9 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK) void dmat_fill_kernel(DmatConst C,
     }
 }
 
-size_t dmat_fill_lds(const DmatConst &) { return FILL_LDS_DOUBLES * sizeof(double); }
+size_t dmat_fill_lds(const DmatConst &C) {
+    return (FILL_LDS_FIXED + (size_t)DMAT_CHUNK_RP * diag_width(C)) * sizeof(double);
+}
 
 hipError_t launch_dmat_pass(const DmatConst &C, const DmatForests &F, const DmatPairs &P,
                             const DmatWork &W, const DmatOut &O, size_t lds,
