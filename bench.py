@@ -474,6 +474,7 @@ def bench_dmat(args):
         log("dmat step %d: %.2f s" % (k, times[-1]))
     sec = float(np.mean(times))
     stats = out[6]
+    rep = eng.dmat_report()
     result = {
         "metric": "distortion-matrix in-range pixel pairs/s (1 GPU)",
         "value": stats[0] / sec, "unit": "pixel pairs/s", "n_gpus": 1, "steps": args.steps,
@@ -485,6 +486,15 @@ def bench_dmat(args):
                    "selected_pairs": int(stats[1]), "bins": "50x50", "model_bins": "100x50",
                    "device": info["name"], "timing": "wall time of picca_dmat_run, "
                    "including the 100 MB download of the matrix"},
+        # the fill kernel on the FP64 matrix cores: the operations it issues there per launch
+        # sequence (16x16x4 products, 2048 FLOP each) over its HIP-event time
+        "roofline": {"bound": "mfma", "unit": "TFLOP/s", "peak": FP64_VALU_PEAK_TFLOPS,
+                     "achieved": rep["mfma_flops"] / (rep["fill_ms"] * 1e-3) / 1e12,
+                     "frac": rep["mfma_flops"] / (rep["fill_ms"] * 1e-3) / 1e12
+                     / FP64_VALU_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "picca::dmat_fill_kernel<false>", "kernel_ms": rep["fill_ms"],
+                     "pass_kernel_ms": rep["pass_ms"], "batches": rep["batches"],
+                     "mfma_flops": rep["mfma_flops"]},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define PICCA_HIP_ABI_VERSION 3
+#define PICCA_HIP_ABI_VERSION 4
 
 #define PICCA_OK 0
 #define PICCA_E_VALUE (-1)   /* invalid argument: picca raises ValueError          */
@@ -83,7 +83,8 @@ typedef struct picca_cf_params {
 /* Counters of one xi execution (no counterpart in picca, which reports none). */
 typedef struct picca_xi_report {
     int64_t accepted_pairs;  /* sum of num_pairs over all rows and bins             */
-    int64_t edge_pairs;      /* pairs within edge_tol of a bin edge or range limit */
+    int64_t edge_pairs;      /* pairs within edge_tol of a bin edge, or outside the range
+                                but within edge_tol of it in both coordinates        */
     int64_t overflow_pairs;  /* bin index >= np*nt (picca writes out of bounds)    */
     int64_t visited_pairs;   /* pixel pairs the kernel evaluated after pruning     */
     int64_t wave_steps;      /* 64-lane pair evaluations issued                     */
@@ -163,9 +164,10 @@ int picca_xi_execute_noalloc(picca_ctx *ctx, const picca_cf_params *params, void
  * r_par_min == 0, no cuts).  RUNS walks each pixel of delta1 through runs of pixels of
  * delta2 that share one bin, summed from per-forest prefix sums, wherever delta2's pixels
  * increase strictly and are spaced well above the rounding margins; PAIRS evaluates every
- * pixel pair.  AUTO (the default) is PAIRS, which measured faster on the synthetic surveys'
- * grids (runs of 2 and 8 pixels).  Results are identical up to FP64 summation order; no
- * counterpart in picca.
+ * pixel pair.  AUTO (the default) walks the tasks whose window of delta2 holds at least
+ * four pixels per r_par bin and takes PAIRS for the others, and for the whole execution
+ * when the store's median pixel step is longer than a quarter bin.  Results are identical
+ * up to FP64 summation order; no counterpart in picca.
  */
 #define PICCA_XI_SCHEDULE_AUTO 0
 #define PICCA_XI_SCHEDULE_PAIRS 1
@@ -213,6 +215,20 @@ int picca_xcf_execute(picca_ctx *ctx, const picca_cf_params *params, picca_xi_re
  * [3] selected pairs whose data bin reaches np*nt by rounding (the reference writes those
  * past the end of weights_dmat; here they are counted and dropped).  stats has 4 entries.
  */
+/*
+ * Device timing and work of the last picca_dmat_run on ctx (no counterpart in picca):
+ * pass_ms / fill_ms the summed HIP-event times of the pair-pass and fill kernels over the
+ * batches, mfma_flops the FP64 operations the fill issued on the matrix cores (2048 per
+ * 16x16x4 instruction).
+ */
+typedef struct picca_dmat_report {
+    double pass_ms;
+    double fill_ms;
+    int64_t batches;
+    int64_t mfma_flops;
+} picca_dmat_report;
+int picca_dmat_last_report(picca_ctx *ctx, picca_dmat_report *report);
+
 int picca_dmat_run(picca_ctx *ctx, const picca_cf_params *params, int64_t num_pairs,
                    const int32_t *pair_a, const int32_t *pair_b, const double *pair_ang,
                    const uint8_t *pair_flags, double *weights_dmat, double *dmat,

@@ -28,8 +28,9 @@
  * Additions that the reference does not have, reported through `stats`:
  *   stats[0] accepted pairs (== sum of num_pairs increments)
  *   stats[1] edge pairs: pairs inside the (r_par, r_trans) range whose non-zero r_par
- *            or r_trans lies within params->edge_tol of a bin edge, plus pairs that
- *            fall outside the range by less than edge_tol
+ *            or r_trans lies within params->edge_tol of a bin edge, plus pairs outside
+ *            the range whose r_par and r_trans both lie within edge_tol of it (a pair
+ *            far outside in one coordinate cannot enter the range whatever the other)
  *   stats[2] overflow pairs: bin index >= np*nt (numba writes out of bounds there;
  *            the oracle skips and counts them)
  */
@@ -122,9 +123,8 @@ void oracle_xi_pair(const oracle_params *p,
                 if (!p->x_correlation) r_par = fabs(r_par);
             }
             if (r_par >= p->r_par_max || r_trans >= p->r_trans_max || r_par < p->r_par_min) {
-                if ((r_par >= p->r_par_max && r_par - p->r_par_max < p->edge_tol) ||
-                    (r_trans >= p->r_trans_max && r_trans - p->r_trans_max < p->edge_tol) ||
-                    (r_par < p->r_par_min && p->r_par_min - r_par < p->edge_tol))
+                if (r_par - p->r_par_max < p->edge_tol && p->r_par_min - r_par < p->edge_tol &&
+                    r_trans - p->r_trans_max < p->edge_tol)
                     stats[1] += 1;
                 continue;
             }
@@ -198,9 +198,8 @@ void oracle_xcf_pair(const oracle_params *p,
                 r_par /= r_trans;
             }
             if (r_par >= p->r_par_max || r_trans >= p->r_trans_max || r_par <= p->r_par_min) {
-                if ((r_par >= p->r_par_max && r_par - p->r_par_max < p->edge_tol) ||
-                    (r_trans >= p->r_trans_max && r_trans - p->r_trans_max < p->edge_tol) ||
-                    (r_par <= p->r_par_min && p->r_par_min - r_par < p->edge_tol))
+                if (r_par - p->r_par_max < p->edge_tol && p->r_par_min - r_par < p->edge_tol &&
+                    r_trans - p->r_trans_max < p->edge_tol)
                     stats[1] += 1;
                 continue;
             }

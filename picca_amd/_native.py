@@ -82,6 +82,19 @@ class XiReport(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class DmatReport(ctypes.Structure):
+    """``picca_dmat_report``."""
+    _fields_ = [
+        ("pass_ms", ctypes.c_double),
+        ("fill_ms", ctypes.c_double),
+        ("batches", ctypes.c_int64),
+        ("mfma_flops", ctypes.c_int64),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class WickReport(ctypes.Structure):
     """``picca_wick_report``."""
     _fields_ = [
@@ -136,6 +149,7 @@ _PROTOTYPES = {
     "picca_xi_use_graph": [_P, _I32],
     "picca_xi_execute_noalloc": [_P, _P, _P, _I64, _P],
     "picca_dmat_run": [_P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "picca_dmat_last_report": [_P, _P],
     "picca_diag_copy": [_P, _I64, _P],
     "picca_cov_subsample": [_P, _I64, _I64, _P, _P, _P, _P],
     "picca_wick_t123": [_P, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P],
@@ -164,7 +178,7 @@ _load_error = None
 
 
 # PICCA_HIP_ABI_VERSION of include/picca_hip.h that these structures mirror
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 
 def exported_symbols():

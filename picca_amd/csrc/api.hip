@@ -109,6 +109,7 @@ int picca_destroy(picca_ctx *ctx) {
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     if (ctx->ev_mid) (void)hipEventDestroy(ctx->ev_mid);
+    for (hipEvent_t e : ctx->dmat_events) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PICCA_OK;
@@ -214,6 +215,18 @@ int picca_set_forests(picca_ctx *ctx, int64_t num_forests, const int64_t *offset
                                                ctx->stream));
         PICCA_HIP_CHECK(ctx->spc.upload(spc.data(), (size_t)num_forests, ctx->stream));
         PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        // typical pixel spacing of the store, for the automatic schedule of the xi kernel
+        std::vector<double> steps;
+        steps.reserve((size_t)num_forests);
+        for (int64_t k = 0; k < num_forests; ++k) {
+            const int64_t o = offsets[k], n = offsets[k + 1] - offsets[k];
+            if (n >= 2) steps.push_back((r_comov[o + n - 1] - r_comov[o]) / (double)(n - 1));
+        }
+        ctx->median_step_r = 0.0;
+        if (!steps.empty()) {
+            std::nth_element(steps.begin(), steps.begin() + steps.size() / 2, steps.end());
+            ctx->median_step_r = steps[steps.size() / 2];
+        }
     }
     PICCA_HIP_CHECK(ctx->zq.upload(z_qso, (size_t)num_forests, ctx->stream));
     PICCA_HIP_CHECK(ctx->mono.upload(mono.data(), (size_t)num_forests, ctx->stream));
@@ -334,7 +347,13 @@ int xi_execute_into(picca_ctx *ctx, const picca::XiConst &C0, int mode,
     picca::XiConst C = C0;
     // the specialised kernel: standard binning, auto-correlation, r_par_min == 0, no cuts
     const bool general = !picca::xi_specialised(params, mode);
-    picca::apply_xi_schedule(C, ctx->xi_schedule);
+    // the automatic schedule keeps the pair loop's kernel (no walk) when the store's pixels
+    // are too dense for runs of XI_WALK_MIN_DENSITY pixels per r_par bin
+    const bool dense = ctx->median_step_r > 0 &&
+                       C.width_p < picca::XI_WALK_MIN_DENSITY * ctx->median_step_r;
+    picca::apply_xi_schedule(C, ctx->xi_schedule == PICCA_XI_SCHEDULE_AUTO && dense
+                                    ? PICCA_XI_SCHEDULE_PAIRS
+                                    : ctx->xi_schedule);
     // LDS: the histogram in its padded layout ((np + 1) x (nt + 1) bins of 48 B) when it
     // fits, plus one staging tile per wave for the loops that stage pixels (the fast loop
     // of the specialised kernel stages none)
@@ -508,6 +527,12 @@ int picca_xi_execute_noalloc(picca_ctx *ctx, const picca_cf_params *params, void
 int picca_xi_use_graph(picca_ctx *ctx, int32_t on) {
     if (!ctx) return fail(PICCA_E_VALUE, "ctx is NULL");
     ctx->xi_use_graph = on != 0;
+    return PICCA_OK;
+}
+
+int picca_dmat_last_report(picca_ctx *ctx, picca_dmat_report *report) {
+    if (!ctx || !report) return fail(PICCA_E_VALUE, "ctx or report is NULL");
+    *report = ctx->dmat_report;
     return PICCA_OK;
 }
 
@@ -920,6 +945,8 @@ int picca_dmat_run(picca_ctx *ctx, const picca_cf_params *params, int64_t num_pa
     C.nt_d = C.nt;
     C.npm_d = C.npm;
     C.ntm_d = C.ntm;
+    C.inv_span_p = 1.0 / C.span_p;
+    C.inv_rtmax = 1.0 / C.rtmax;
     // LDS of the pass kernel: U bitmap, row ranges, then bands of k r_trans rows
     // (4 sums per data bin, 8 per model bin)
     const size_t fixed = ((size_t)C.words_u * 4 + (size_t)(4 * C.nt + 4 * C.ntm) * 4 + 15) &
@@ -1002,6 +1029,7 @@ int picca_dmat_run(picca_ctx *ctx, const picca_cf_params *params, int64_t num_pa
     picca::DmatOut O{ctx->o_wdm.ptr, ctx->o_dmat.ptr, ctx->o_eff.ptr, ctx->o_eff.ptr + nm,
                      ctx->o_eff.ptr + 2 * nm, ctx->o_eff.ptr + 3 * nm};
     std::vector<int64_t> pix_off;
+    int64_t batches = 0;
     for (int64_t p0 = 0; p0 < num_pairs; p0 += cap) {
         const int64_t count = std::min(cap, num_pairs - p0);
         pix_off.assign((size_t)count, 0);
@@ -1029,12 +1057,21 @@ int picca_dmat_run(picca_ctx *ctx, const picca_cf_params *params, int64_t num_pa
                           ctx->ws_pix.ptr,
                           (uint32_t *)ctx->ws_touched.ptr,
                           ctx->dstats.ptr};
+        const size_t ev = 3 * (size_t)batches++;
+        while (ctx->dmat_events.size() < ev + 3) {
+            hipEvent_t e = nullptr;
+            PICCA_HIP_CHECK(hipEventCreate(&e));
+            ctx->dmat_events.push_back(e);
+        }
+        PICCA_HIP_CHECK(hipEventRecord(ctx->dmat_events[ev], ctx->stream));
         PICCA_HIP_CHECK(picca::launch_dmat_pass(C, F, P, W, O, pass_lds, ctx->stream));
+        PICCA_HIP_CHECK(hipEventRecord(ctx->dmat_events[ev + 1], ctx->stream));
         const int groups = (int)std::max<int64_t>(1, std::min<int64_t>(32, count / 16));
         bool quasar = true;   // every delta2 of the batch unprojected
         for (int64_t k = 0; k < count && quasar; ++k)
             quasar = ctx->h_order[(size_t)pair_b[p0 + k]] < 0;
         PICCA_HIP_CHECK(picca::launch_dmat_fill(C, F, P, W, O, groups, quasar, ctx->stream));
+        PICCA_HIP_CHECK(hipEventRecord(ctx->dmat_events[ev + 2], ctx->stream));
     }
 
     std::vector<double> h_wdm((size_t)nb), h_dmat((size_t)(nb * nm)), h_eff((size_t)(4 * nm));
@@ -1048,6 +1085,17 @@ int picca_dmat_run(picca_ctx *ctx, const picca_cf_params *params, int64_t num_pa
     PICCA_HIP_CHECK(hipMemcpyAsync(h_stats, ctx->dstats.ptr, sizeof(h_stats),
                                    hipMemcpyDeviceToHost, ctx->stream));
     PICCA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->dmat_report = picca_dmat_report{0.0, 0.0, batches,
+                                         (int64_t)h_stats[picca::DMAT_STAT_MFMA] * 2048};
+    for (int64_t k = 0; k < batches; ++k) {
+        float pass_ms = 0.f, fill_ms = 0.f;
+        PICCA_HIP_CHECK(hipEventElapsedTime(&pass_ms, ctx->dmat_events[3 * k],
+                                            ctx->dmat_events[3 * k + 1]));
+        PICCA_HIP_CHECK(hipEventElapsedTime(&fill_ms, ctx->dmat_events[3 * k + 1],
+                                            ctx->dmat_events[3 * k + 2]));
+        ctx->dmat_report.pass_ms += pass_ms;
+        ctx->dmat_report.fill_ms += fill_ms;
+    }
     if (h_stats[picca::DMAT_STAT_NEGATIVE] > 0)
         return fail(PICCA_E_INDEX, "a selected pair has a negative data bin index");
     for (int64_t k = 0; k < nb; ++k) weights_dmat[k] += h_wdm[k];

@@ -102,6 +102,10 @@ struct picca_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_mid = nullptr;
+    // distortion matrix: events around each batch's pass and fill kernels (a pool that
+    // grows with the batches of a run) and the last run's report
+    std::vector<hipEvent_t> dmat_events;
+    picca_dmat_report dmat_report = {0.0, 0.0, 0, 0};
     int num_cus = 0;
     int64_t lds_limit = 0;
     unsigned long long *scratch = nullptr;   // this GPU's no-allocation scratch array
@@ -125,6 +129,7 @@ struct picca_ctx {
     DeviceBuffer<double2> rd, spc;
     DeviceBuffer<picca::PrefixRecord> pre;
     int32_t xi_schedule = 0;   // PICCA_XI_SCHEDULE_*
+    double median_step_r = 0;  // median over forests of the mean r_comov step between pixels
     // the xi launch sequence (zero kernels + pair kernel) as a HIP graph, replayed while
     // its launch arguments stay the same
     bool xi_use_graph = true;

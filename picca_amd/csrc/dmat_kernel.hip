@@ -105,11 +105,26 @@ __device__ __forceinline__ bool zerr_cut(const DmatConst &C, double z, double z_
     return dv_kms < C.zerr_kms;
 }
 
-// Separations, range test and data/model bins of one pixel pair with the reference's
-// arithmetic (py/picca/cf.py:656-700).  A and M are the flat bin indices.
+// Data and model bins of one in-range pixel pair: the flat indices A = bt + nt bp and
+// M = mt + ntm mp of the reference, and their row (r_par) and column (r_trans) parts as the
+// reference's flat index reads them (a column bin that rounds to nt is column 0 of the next
+// row).
+struct PairBins {
+    int A, M;
+    int tA, rA;   // A % nt, A / nt
+    int uM, mM;   // M % ntm, M / ntm
+};
+
+// Separations, range test and data/model bins of one pixel pair (py/picca/cf.py:656-700).
+// The bin coordinates x np, y nt, x npm, y ntm (x = (r_par - r_par_min) / span, y =
+// r_trans / r_trans_max) are taken with reciprocal multiplies, which differ from the
+// reference's divisions by a few ulp; a coordinate t within 1e-12 (1 + t) of an integer
+// (far above that difference) is redone with the divisions, so every pair gets the
+// reference's bins.
+// An exact zero is the same in both forms.
 __device__ __forceinline__ bool pair_bins(const DmatConst &C, double r1, double d1, double r2,
                                           double d2, double cos_half, double sin_half,
-                                          double &r_par, double &r_trans, int &A, int &M) {
+                                          double &r_par, double &r_trans, PairBins &B) {
     r_par = (r1 - r2) * cos_half;
     r_trans = (d1 + d2) * sin_half;
     if (C.flags & DMAT_RMU) {
@@ -118,14 +133,30 @@ __device__ __forceinline__ bool pair_bins(const DmatConst &C, double r1, double 
     }
     if (!(C.flags & DMAT_XCORR)) r_par = fabs(r_par);
     if (r_par >= C.rpmax || r_trans >= C.rtmax || r_par < C.rpmin) return false;
-    const double x = (r_par - C.rpmin) / C.span_p;
-    const double y = r_trans / C.rtmax;
-    const double bp = floor(x * C.np_d);
-    const double bt = floor(y * C.nt_d);
-    A = (int)(bt + C.nt_d * bp);
-    const double mp = floor(x * C.npm_d);
-    const double mt = floor(y * C.ntm_d);
-    M = (int)(mt + C.ntm_d * mp);
+    double x = (r_par - C.rpmin) * C.inv_span_p;
+    double y = r_trans * C.inv_rtmax;
+    double tp = x * C.np_d, tt = y * C.nt_d, up = x * C.npm_d, ut = y * C.ntm_d;
+    // the two forms differ by a few ulp of the coordinate (< 1e-15 t); t >= 0
+    auto near_int = [](double t) { return fabs(t - rint(t)) < 1e-12 * (1.0 + t); };
+    const bool near = (x != 0 && (near_int(tp) || near_int(up))) ||
+                      (y != 0 && (near_int(tt) || near_int(ut)));
+    if (near) {
+        x = (r_par - C.rpmin) / C.span_p;
+        y = r_trans / C.rtmax;
+        tp = x * C.np_d;
+        tt = y * C.nt_d;
+        up = x * C.npm_d;
+        ut = y * C.ntm_d;
+    }
+    const double bp = floor(tp), bt = floor(tt), mp = floor(up), mt = floor(ut);
+    B.A = (int)(bt + C.nt_d * bp);
+    B.M = (int)(mt + C.ntm_d * mp);
+    // bt <= nt and mt <= ntm (y < 1 up to rounding); x >= 0 and y >= 0
+    const bool wrap_t = bt >= C.nt_d, wrap_u = mt >= C.ntm_d;
+    B.tA = wrap_t ? 0 : (int)bt;
+    B.rA = (int)bp + (wrap_t ? 1 : 0);
+    B.uM = wrap_u ? 0 : (int)mt;
+    B.mM = (int)mp + (wrap_u ? 1 : 0);
     return true;
 }
 
@@ -466,10 +497,11 @@ __global__ __launch_bounds__(DMAT_PASS_BLOCK) void dmat_pass_kernel(DmatConst C,
                 const double f2 = evol ? Bf[j] : 1.0;
                 const bool j_zerr = q.zerr_on && zerr_cut(C, z2, q.zq1);
                 double r_par, r_trans;
-                int A = 0, M = 0;
+                PairBins B;
                 if (!active || !pair_bins(C, r1, d1, r2, d2, q.cos_half, q.sin_half, r_par,
-                                          r_trans, A, M))
+                                          r_trans, B))
                     continue;
+                const int A = B.A, M = B.M;
                 ++n_in;   // the count pass takes every in-range pair here
                 if (M < 0 || M >= C.nm) continue;
                 const double z = (z1 + z2) / 2;
@@ -477,7 +509,7 @@ __global__ __launch_bounds__(DMAT_PASS_BLOCK) void dmat_pass_kernel(DmatConst C,
                     atomicOr(&lU[M >> 5], 1u << (M & 31));
                     last_m = M;
                 }
-                row_step(ru, M % C.ntm, j, l_iu, l_ju, i);
+                row_step(ru, B.uM, j, l_iu, l_ju, i);
                 const double w12 = w1 * w2;
                 const double zfac = evol ? f1 * f2 : 1.0;
                 const bool sel = pair_selected(C, q, i_sel, j_zerr, z, r_par);
@@ -505,7 +537,7 @@ __global__ __launch_bounds__(DMAT_PASS_BLOCK) void dmat_pass_kernel(DmatConst C,
                     ++n_over;
                     continue;
                 }
-                row_step(rt, A % C.nt, j, l_it, l_jt, i);
+                row_step(rt, B.tA, j, l_it, l_jt, i);
                 if (A != ra.key) {
                     if (ra.key >= 0) flush_a(ra, g_t0);
                     ra = RunA{A, 0, 0, 0, 0};
@@ -561,10 +593,11 @@ __global__ __launch_bounds__(DMAT_PASS_BLOCK) void dmat_pass_kernel(DmatConst C,
                 const double r2 = Br[j], d2 = Bd[j], z2 = Bz[j];
                 const bool j_zerr = q.zerr_on && zerr_cut(C, z2, q.zq1);
                 double r_par, r_trans;
-                int A = 0, M = 0;
+                PairBins B;
                 if (!active || !pair_bins(C, r1, d1, r2, d2, q.cos_half, q.sin_half, r_par,
-                                          r_trans, A, M))
+                                          r_trans, B))
                     continue;
+                const int A = B.A, M = B.M;
                 ++n_in;
                 ++cnt_in_i;
                 const bool close = q.shp_cut && fabs(r_par) < C.width_p;
@@ -577,7 +610,7 @@ __global__ __launch_bounds__(DMAT_PASS_BLOCK) void dmat_pass_kernel(DmatConst C,
                     atomicOr(&lU[M >> 5], 1u << (M & 31));
                     last_m = M;
                 }
-                row_step(ru, M % C.ntm, j, l_iu, l_ju, i);
+                row_step(ru, B.uM, j, l_iu, l_ju, i);
                 if (!pair_selected(C, q, i_sel, j_zerr, z, r_par)) continue;
                 ++n_sel;
                 if (A < 0) {
@@ -588,7 +621,7 @@ __global__ __launch_bounds__(DMAT_PASS_BLOCK) void dmat_pass_kernel(DmatConst C,
                     ++n_over;
                     continue;
                 }
-                row_step(rt, A % C.nt, j, l_it, l_jt, i);
+                row_step(rt, B.tA, j, l_it, l_jt, i);
             }
             if (rt.row >= 0) note_run(l_it, l_jt, rt.row, i, rt.first, rt.last);
             if (ru.row >= 0) note_run(l_iu, l_ju, ru.row, i, ru.first, ru.last);
@@ -707,10 +740,11 @@ __global__ __launch_bounds__(DMAT_PASS_BLOCK) void dmat_pass_kernel(DmatConst C,
                 const double w2 = Bw[j];
                 if (w2 == 0) continue;
                 double r_par, r_trans;
-                int A = 0, M = 0;
+                PairBins B;
                 if (!active || !pair_bins(C, r1, d1, Br[j], Bd[j], q.cos_half, q.sin_half,
-                                          r_par, r_trans, A, M))
+                                          r_par, r_trans, B))
                     continue;
+                const int A = B.A, M = B.M;
                 if (g < (int64_t)count_pass && M >= 0 && M < C.nm)
                     atomicOr(&lU[M >> 5], 1u << (M & 31));
                 ++g;
@@ -763,10 +797,11 @@ __global__ __launch_bounds__(DMAT_PASS_BLOCK) void dmat_pass_kernel(DmatConst C,
                 const double f2 = evol ? Bf[j] : 1.0;
                 const bool j_zerr = q.zerr_on && zerr_cut(C, z2, q.zq1);
                 double r_par, r_trans;
-                int A = 0, M = 0;
+                PairBins B;
                 if (!active || !pair_bins(C, r1, d1, r2, d2, q.cos_half, q.sin_half, r_par,
-                                          r_trans, A, M))
+                                          r_trans, B))
                     continue;
+                const int A = B.A, M = B.M;
                 if (M < 0 || M >= C.nm) continue;
                 const double z = (z1 + z2) / 2;
                 const double w12 = w1 * w2;
@@ -892,6 +927,7 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK, 3) void dmat_fill_kernel(DmatConst
     const bool evol = (C.flags & DMAT_REDSHIFT_EVOL) != 0;
 
     const int nd = diag_width(C);
+    uint32_t n_mfma = 0;   // matrix-core instructions of this wave (the statistic)
     double *diag = (double *)smem;                              // [CHUNK_RP][nd]
     double *vS = diag + DMAT_CHUNK_RP * nd;                     // [4][CHUNK_RP]
     double *vE = vS + 4 * DMAT_CHUNK_RP;                        // [4][CHUNK_MRP]
@@ -930,6 +966,7 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK, 3) void dmat_fill_kernel(DmatConst
                 const double a2 = kin ? -v[VEC_P2 + row] : 0.0;
                 // no pair of these pixels in this wave's data bins: nothing to add
                 if (__ballot(a1 != 0.0 || a2 != 0.0) == 0) return;
+                n_mfma += 2 * FILL_CB;
 #pragma unroll
                 for (int b = 0; b < FILL_CB; ++b) {
                     const double e1 = kin ? v[VEC_E + 16 * b + li] : 0.0;
@@ -975,6 +1012,7 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK, 3) void dmat_fill_kernel(DmatConst
                     // rank-4 term: one K = 4 product per block (K = the four sums)
                     const double sa = vS[lk * DMAT_CHUNK_RP + row];
                     if (__ballot(sa != 0.0) != 0) {
+                        n_mfma += FILL_CB;
 #pragma unroll
                         for (int b = 0; b < FILL_CB; ++b)
                             acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(
@@ -1021,15 +1059,16 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK, 3) void dmat_fill_kernel(DmatConst
                                     const double w2 = Bw[j];
                                     if (w2 == 0) continue;
                                     double r_par, r_trans;
-                                    int A = 0, M = 0;
+                                    PairBins B;
                                     if (!pair_bins(C, r1, d1, Br[j], Bd[j], q.cos_half,
-                                                   q.sin_half, r_par, r_trans, A, M))
+                                                   q.sin_half, r_par, r_trans, B))
                                         continue;
+                                    const int A = B.A, M = B.M;
                                     if (M < 0 || M >= C.nm) continue;
                                     const double z2 = Bz[j], l2 = Bl[j];
                                     const double zfac = evol ? f1 * Bf[j] : 1.0;
-                                    const int mu = M % C.ntm;
-                                    const int m = M / C.ntm - mrp0;
+                                    const int mu = B.uM;
+                                    const int m = B.mM - mrp0;
                                     const bool in_m = mu == u && m >= 0 && m < nmrp;
                                     if (in_m && mask[m] != 0 && q.proj2) {
                                         lds_add(vec + VEC_E + m, zfac * w2 * q.inv_w2);
@@ -1039,8 +1078,8 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK, 3) void dmat_fill_kernel(DmatConst
                                     const bool j_zerr = q.zerr_on && zerr_cut(C, z2, q.zq1);
                                     if (!pair_selected(C, q, i_sel, j_zerr, (z1 + z2) / 2, r_par))
                                         continue;
-                                    if (A < 0 || A >= C.nb || A % C.nt != t) continue;
-                                    const int r = A / C.nt - rp0;
+                                    if (A < 0 || A >= C.nb || B.tA != t) continue;
+                                    const int r = B.rA - rp0;
                                     if (r < 0 || r >= nrp) continue;
                                     const double w12 = w1 * w2;
                                     lds_add(vec + VEC_P + r, w12);
@@ -1080,15 +1119,16 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK, 3) void dmat_fill_kernel(DmatConst
                             const double w1 = Aw[i];
                             if (w1 == 0) continue;
                             double r_par, r_trans;
-                            int A = 0, M = 0;
+                            PairBins B;
                             if (!pair_bins(C, Ar[i], Ad[i], r2, d2, q.cos_half, q.sin_half,
-                                           r_par, r_trans, A, M))
+                                           r_par, r_trans, B))
                                 continue;
+                            const int A = B.A, M = B.M;
                             if (M < 0 || M >= C.nm) continue;
                             const double z1 = Az[i], l1 = Al[i];
                             const double zfac = evol ? Af[i] * f2 : 1.0;
-                            const int mu = M % C.ntm;
-                            const int m = M / C.ntm - mrp0;
+                            const int mu = B.uM;
+                            const int m = B.mM - mrp0;
                             const bool in_m = mu == u && m >= 0 && m < nmrp;
                             if (in_m && mask[m] != 0) {
                                 lds_add(sv + VEC_E + m, zfac * w1 * q.inv_w1);
@@ -1097,8 +1137,8 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK, 3) void dmat_fill_kernel(DmatConst
                             const bool i_sel = !(q.zerr_on && zerr_cut(C, z1, q.zq2));
                             if (!pair_selected(C, q, i_sel, j_zerr, (z1 + z2) / 2, r_par))
                                 continue;
-                            if (A < 0 || A >= C.nb || A % C.nt != t) continue;
-                            const int r = A / C.nt - rp0;
+                            if (A < 0 || A >= C.nb || B.tA != t) continue;
+                            const int r = B.rA - rp0;
                             if (r < 0 || r >= nrp) continue;
                             const double w12 = w1 * w2;
                             lds_add(sv + VEC_P + r, w12);
@@ -1137,15 +1177,16 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK, 3) void dmat_fill_kernel(DmatConst
                                     const double w1 = Aw[i];
                                     if (w1 == 0) continue;
                                     double r_par, r_trans;
-                                    int A = 0, M = 0;
+                                    PairBins B;
                                     if (!pair_bins(C, Ar[i], Ad[i], r2, d2, q.cos_half,
-                                                   q.sin_half, r_par, r_trans, A, M))
+                                                   q.sin_half, r_par, r_trans, B))
                                         continue;
+                                    const int A = B.A, M = B.M;
                                     if (M < 0 || M >= C.nm) continue;
                                     const double z1 = Az[i], l1 = Al[i];
                                     const double zfac = evol ? Af[i] * f2 : 1.0;
-                                    const int mu = M % C.ntm;
-                                    const int m = M / C.ntm - mrp0;
+                                    const int mu = B.uM;
+                                    const int m = B.mM - mrp0;
                                     if (mu == u && m >= 0 && m < nmrp && mask[m] != 0) {
                                         lds_add(vec + VEC_E + m, zfac * w1 * q.inv_w1);
                                         if (q.order1)
@@ -1154,8 +1195,8 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK, 3) void dmat_fill_kernel(DmatConst
                                     const bool i_sel = !(q.zerr_on && zerr_cut(C, z1, q.zq2));
                                     if (!pair_selected(C, q, i_sel, j_zerr, (z1 + z2) / 2, r_par))
                                         continue;
-                                    if (A < 0 || A >= C.nb || A % C.nt != t) continue;
-                                    const int r = A / C.nt - rp0;
+                                    if (A < 0 || A >= C.nb || B.tA != t) continue;
+                                    const int r = B.rA - rp0;
                                     if (r < 0 || r >= nrp) continue;
                                     const double w12 = w1 * w2;
                                     lds_add(vec + VEC_P + r, w12);
@@ -1194,6 +1235,7 @@ __global__ __launch_bounds__(DMAT_FILL_BLOCK, 3) void dmat_fill_kernel(DmatConst
             __syncthreads();
         }
     }
+    if (lane == 0 && n_mfma) atomicAdd(&W.stats[DMAT_STAT_MFMA], (unsigned long long)n_mfma);
 }
 
 size_t dmat_fill_lds(const DmatConst &C) {

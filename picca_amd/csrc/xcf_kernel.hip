@@ -59,9 +59,9 @@ __device__ __forceinline__ bool xcf_pair(const XiConst &C, const PixelRecord &q,
         r_par /= r_trans;
     }
     if (r_par >= C.rpmax || r_trans >= C.rtmax || r_par <= C.rpmin) {
-        if ((r_par >= C.rpmax && r_par - C.rpmax < C.edge_tol) ||
-            (r_trans >= C.rtmax && r_trans - C.rtmax < C.edge_tol) ||
-            (r_par <= C.rpmin && C.rpmin - r_par < C.edge_tol))
+        // outside the range, but within edge_tol of it in both coordinates
+        if (r_par - C.rpmax < C.edge_tol && C.rpmin - r_par < C.edge_tol &&
+            r_trans - C.rtmax < C.edge_tol)
             ++edge;
         return false;
     }
@@ -198,9 +198,7 @@ __device__ __forceinline__ int exact_xcf(const XiConst &K, double r1, double d1,
     const double r_par = (r1 - r2) * c;
     const double r_trans = (d1 + d2) * s;
     const bool out = (r_par >= rpmax) | (r_trans >= rtmax) | (r_par <= rpmin);
-    const bool edge_out = ((r_par >= rpmax) & (r_par - rpmax < tol)) |
-                          ((r_trans >= rtmax) & (r_trans - rtmax < tol)) |
-                          ((r_par <= rpmin) & (rpmin - r_par < tol));
+    const bool edge_out = (r_par - rpmax < tol) & (rpmin - r_par < tol) & (r_trans - rtmax < tol);
     const double t_par = (r_par - rpmin) / span * np_d;
     const double t_trans = r_trans / rtmax * nt_d;
     const double f_par = floor(t_par);

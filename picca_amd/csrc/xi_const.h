@@ -82,13 +82,13 @@ inline const char *widen_xi_margins(XiConst &C, double max_abs_r, double max_abs
     return nullptr;
 }
 
-// Schedule of the specialised kernel (PICCA_XI_SCHEDULE_*).  The automatic choice is the
-// pair loop: on the 0.8 A grid of the 100k survey (about 8 pixel pairs per run) the run walk
-// measured 36.6 s against the pair loop's 31.1 s, and 12.4 ms against 4.6 ms on the
-// log10-lambda grid (about 2 pairs per run).
+// Schedule of the specialised kernel (PICCA_XI_SCHEDULE_*): RUNS walks every eligible task,
+// AUTO the eligible tasks whose window holds at least XI_WALK_MIN_DENSITY pixels per r_par
+// bin (walk_task in xi_walk.h), PAIRS none.
 inline void apply_xi_schedule(XiConst &C, int schedule) {
+    C.flags &= ~(XI_WALK | XI_WALK_AUTO);
     if (schedule == PICCA_XI_SCHEDULE_RUNS) C.flags |= XI_WALK;
-    else C.flags &= ~XI_WALK;
+    else if (schedule == PICCA_XI_SCHEDULE_AUTO) C.flags |= XI_WALK | XI_WALK_AUTO;
 }
 
 // The specialised kernel (and its run walk) covers standard binning, auto-correlation,

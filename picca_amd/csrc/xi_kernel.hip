@@ -80,10 +80,10 @@ __device__ __forceinline__ bool exact_pair(const XiConst &C, double r1, double d
         if (!XCORR) r_par = fabs(r_par);
     }
     if (r_par >= C.rpmax || r_trans >= C.rtmax || r_par < C.rpmin) {
-        // pairs just outside the range are edge pairs too
-        if ((r_par >= C.rpmax && r_par - C.rpmax < C.edge_tol) ||
-            (r_trans >= C.rtmax && r_trans - C.rtmax < C.edge_tol) ||
-            (r_par < C.rpmin && C.rpmin - r_par < C.edge_tol))
+        // pairs just outside the range (within edge_tol of it in both coordinates) are edge
+        // pairs too
+        if (r_par - C.rpmax < C.edge_tol && C.rpmin - r_par < C.edge_tol &&
+            r_trans - C.rtmax < C.edge_tol)
             ++edge;
         return false;
     }
@@ -131,8 +131,8 @@ __device__ __forceinline__ int exact_fast(const XiConst &K, double r1, double d1
     const double r_par = fabs((r1 - r2) * cos_half);
     const double r_trans = (d1 + d2) * sin_half;
     const bool out = (r_par >= rpmax) | (r_trans >= rtmax);
-    const bool edge_out = ((r_par >= rpmax) & (r_par - rpmax < tol)) |
-                          ((r_trans >= rtmax) & (r_trans - rtmax < tol));
+    // outside the range, but within edge_tol of it in both coordinates
+    const bool edge_out = (r_par - rpmax < tol) & (r_trans - rtmax < tol);
     const double t_par = (r_par - 0.0) / span * np_d;
     const double t_trans = r_trans / rtmax * nt_d;
     const double f_par = floor(t_par);
@@ -328,12 +328,13 @@ __device__ void xi_task(const XiConst &C, const XiConst &C_lds, const ForestView
     }
     j_lo = uniform_int(j_lo);
     j_hi = uniform_int(j_hi);
+    // an empty window (about half the tasks of a survey): no pair of this chunk is in range
+    if (j_lo >= j_hi) return;
 
     if constexpr (!GENERAL && LDS_HIST) {
         if (WALK && (C.flags & XI_WALK) && pruned) {
-            // Run walk (xi_walk.h) when delta2's pixels are strictly increasing in r_comov
-            // and dist_m and consecutive pixels lie more than four rounding margins apart
-            // in both bin coordinates; otherwise the pair loop below.
+            // Run walk (xi_walk.h) for the tasks walk_task admits; otherwise the pair loop
+            // below.
             const double spc_r = uniform_f64(F.spc[b].x);
             const double spc_d = uniform_f64(F.spc[b].y);
             const double cp = cos_half * C.inv_p;
@@ -344,8 +345,7 @@ __device__ void xi_task(const XiConst &C, const XiConst &C_lds, const ForestView
                                      uniform_f64(F.rd[ob + j_lo].x)) * cp;
                 if (span > 0) q0 = (double)(j_hi - 1 - j_lo) / span;
             }
-            if (spc_r > 0 && spc_d > 0 && cp * spc_r > 4.0 * (0.5 - C.half_p) &&
-                st * spc_d > 4.0 * (0.5 - C.half_t)) {
+            if (walk_task(C, spc_r, spc_d, cp, st, q0)) {
                 WalkTask T;
                 T.cos_half = cos_half;
                 T.sin_half = sin_half;
@@ -356,18 +356,18 @@ __device__ void xi_task(const XiConst &C, const XiConst &C_lds, const ForestView
                 T.j_lo = j_lo;
                 T.j_hi = j_hi;
                 T.q0 = q0;
-                T.r_lo = j_hi > j_lo ? uniform_f64(F.rd[ob + j_lo].x) : 0.0;
                 DeviceWalkMem mem{F.rd + ob, F.pre + (ob + b), F.w + ob, F.z + ob, F.dw + ob,
                                   hist, lds_stats, 0u, 0u, 0.0};
                 if (active) {
-                    const WalkLane L{r1, d1, zh1, w1, dw1, (lane + 0.5) * (1.0 / 64.0)};
+                    const WalkLane L{r1, d1, zh1, w1, dw1};
                     walk_lane(C, T, L, mem);
                 }
                 cnt.visit += (uint64_t)mem.cover;
                 cnt.runs += mem.runs;
                 cnt.probes += mem.probes;
                 ++n_walk;
-                if (j_hi > j_lo) ++n_steps;
+                // wave passes of the walk: the lanes' loops run until the longest is done
+                n_steps += (uint64_t)uniform_int(wave_max_int((int)mem.probes));
                 return;
             }
         }

@@ -7,10 +7,10 @@
 // So the pairs of pixel i split into runs of consecutive j with one (side, r_par bin,
 // r_trans bin) key, and a run's contribution to compute_xi's five sums and the pair count
 // is the difference of two entries of prefix sums over delta2.  The walk finds each run's
-// end with a few probes (a guess from the local pixel spacing, then steps and bisection),
-// so its cost grows with the number of runs, not of pairs.
+// end with a few probes (a guess from the window's pixel density, then steps and
+// bisection), so its cost grows with the number of runs, not of pairs.
 //
-// Exactness: a probe takes the bin coordinates with a reciprocal multiply and redoes the
+// Exactness: a probe takes the bin coordinates with reciprocal multiplies and redoes the
 // pixel with the reference's arithmetic when a coordinate lies within the rounding margin
 // of an integer, so every probed key is the reference's.  Keys are monotone, so the pixels
 // between two probes with one key share it.  A pixel within the margin of a bin edge or
@@ -83,29 +83,24 @@ struct WalkTask {
     double cos_half, sin_half;
     double cp, st;       // cos(ang/2) * np / span_p and sin(ang/2) * nt / r_trans_max
     double rc, dc;       // r_c and d_c of delta2's prefix sums
-    double q0;           // first estimate of delta2 pixels per unit of the r_par coordinate
-    double r_lo;         // r_comov of delta2's pixel j_lo
+    double q0;           // mean delta2 pixels per unit of the r_par coordinate in the window
     int32_t j_lo, j_hi;  // pruned pixel window of delta2
 };
 
 // Pixel i of delta1 (weight != 0).
 struct WalkLane {
     double r1, d1, zh1, w1, dw1;  // zh1 = z1 / 2, dw1 = delta1 * w1
-    double phase;                 // in [0, 1): where in its r_par range the walk starts
-};
-
-// Key of a pixel pair: m is the signed r_par run index, m = bin for r_comov2 >= r_comov1 and
-// -bin - 1 below it, clamped to [-np - 1, np] (|m| beyond np - 1 is rejected); c is the
-// r_trans bin clamped to nt (nt is rejected, or a wrap-around pair).
-struct WalkKey {
-    int32_t m, c;
 };
 
 // Everything the reference does with one pixel pair (standard binning, auto-correlation,
-// r_par_min == 0, no cuts), py/picca/cf.py:355-376, with its key.
+// r_par_min == 0, no cuts), py/picca/cf.py:355-376, with its key: m is the signed r_par run
+// index, m = bin for r_comov2 >= r_comov1 and -bin - 1 below it, clamped to [-np - 1, np]
+// (|m| beyond np - 1 is rejected); c is the r_trans bin clamped to nt (nt is rejected, or a
+// wrap-around pair).
 struct WalkExact {
     int32_t m, c;
-    bool edge;      // within edge_tol of a bin edge or of a range limit
+    bool edge;      // within edge_tol of a bin edge, or outside the range but within
+                    // edge_tol of it in both coordinates
     bool over;      // accepted, flat bin index >= np * nt (picca writes out of bounds)
     bool wrap;      // accepted with its r_trans bin rounded to nt: the flat index is
                     // column 0 of the next r_par row, where the pair is added
@@ -136,8 +131,8 @@ PICCA_HD inline WalkExact walk_exact(const XiConst &C, double r1, double d1, dou
     const bool out_p = r_par >= C.rpmax;
     const bool out_t = r_trans >= C.rtmax;
     const bool out = out_p | out_t;
-    const bool edge_out = (out_p & (r_par - C.rpmax < C.edge_tol)) |
-                          (out_t & (r_trans - C.rtmax < C.edge_tol));
+    // outside the range, but within edge_tol of it in both coordinates
+    const bool edge_out = (r_par - C.rpmax < C.edge_tol) & (r_trans - C.rtmax < C.edge_tol);
     const bool edge_in = (r_par != 0 && walk_edge_distance(t_par, C.width_p) < C.edge_tol) ||
                          (r_trans != 0 && walk_edge_distance(t_trans, C.width_t) < C.edge_tol);
     e.edge = out ? edge_out : edge_in;
@@ -148,35 +143,6 @@ PICCA_HD inline WalkExact walk_exact(const XiConst &C, double r1, double d1, dou
     e.r_par = r_par;
     e.r_trans = r_trans;
     return e;
-}
-
-// Key from the reciprocal-multiply coordinates; `near` flags a coordinate within the
-// rounding margin of an integer, where only walk_exact gives the reference's key.
-PICCA_HD inline WalkKey walk_fast_key(const XiConst &C, const WalkTask &T, double r1, double d1,
-                                      double r2, double d2, double &t_par, bool &near) {
-    t_par = (r2 - r1) * T.cp;
-    const double t_tr = (d1 + d2) * T.st;
-    const double fp = floor(t_par);
-    const double ft = floor(t_tr);
-    near = fabs(t_par - fp - 0.5) > C.half_p || fabs(t_tr - ft - 0.5) > C.half_t;
-    WalkKey k;
-    k.m = (int32_t)fmin(fmax(fp, -C.np_d - 1.0), C.np_d);
-    k.c = (int32_t)fmin(ft, C.nt_d);
-    return k;
-}
-
-template <class Mem>
-PICCA_HD inline WalkKey walk_key(const XiConst &C, const WalkTask &T, const WalkLane &L,
-                                 Mem &mem, int p, double &t_par, bool &near) {
-    double r2, d2;
-    mem.rd(p, r2, d2);
-    WalkKey k = walk_fast_key(C, T, L.r1, L.d1, r2, d2, t_par, near);
-    if (near) {
-        const WalkExact e = walk_exact(C, L.r1, L.d1, r2, d2, T.cos_half, T.sin_half);
-        k.m = e.m;
-        k.c = e.c;
-    }
-    return k;
 }
 
 // The reference's treatment of a pixel within the margin of an edge: edge and overflow
@@ -199,13 +165,13 @@ PICCA_HD inline void walk_special(const XiConst &C, const WalkTask &T, const Wal
     }
 }
 
-// Adds the run [j, jn) of key k: prefix differences times the lane's pixel.
+// Adds the run [j, jn) of key (m, c): prefix differences times the lane's pixel.
 template <class Mem>
 PICCA_HD inline void walk_flush(const XiConst &C, const WalkTask &T, const WalkLane &L, Mem &mem,
-                                WalkKey k, const PrefixRecord &ps, const PrefixRecord &pe) {
+                                int m, int c, const PrefixRecord &ps, const PrefixRecord &pe) {
     const double n = pe.c - ps.c;
     if (!(n > 0)) return;
-    const int row = k.m >= 0 ? k.m : -k.m - 1;
+    const int row = m >= 0 ? m : -m - 1;
     const double dw = pe.w - ps.w;
     const double dx = pe.x - ps.x;
     const double dr = pe.r - ps.r;
@@ -214,135 +180,176 @@ PICCA_HD inline void walk_flush(const XiConst &C, const WalkTask &T, const WalkL
     // sum over the run of w2 (r2 - r1) and w2 (d1 + d2), from the centred sums
     const double s_r = dr - (L.r1 - T.rc) * dw;
     const double s_d = (L.d1 + T.dc) * dw + dd;
-    const double rp = T.cos_half * (L.w1 * (k.m >= 0 ? s_r : -s_r));
+    const double rp = T.cos_half * (L.w1 * (m >= 0 ? s_r : -s_r));
     const double rt = T.sin_half * (L.w1 * s_d);
     const double z = L.w1 * (L.zh1 * dw + 0.5 * dz);
-    mem.add(row * C.ntp + k.c, L.w1 * dw, L.dw1 * dx, rp, rt, z, (uint32_t)n);
+    mem.add(row * C.ntp + c, L.w1 * dw, L.dw1 * dx, rp, rt, z, (uint32_t)n);
 }
 
-// First guess of the end of the run that starts at pixel j with key k: the pixel where the
-// r_par coordinate reaches the run's upper edge at q pixels per unit, in (j, j_end].
-PICCA_HD inline int walk_guess(WalkKey k, double t_j, double q, int j, int j_end) {
-    double ahead = ceil(((double)(k.m + 1) - t_j) * q);
-    ahead = fmin(fmax(ahead, 1.0), (double)(j_end - j));
-    return j + (int)ahead;
+// Key of pixel p from its coordinates relative to the upper edges of the run key (m, c):
+// a = (r2 - r1) cp - (m + 1) and b = (d1 + d2) st - (c + 1), so the pixel's r_par run index
+// is m + 1 + floor(a) and its r_trans bin c + 1 + floor(b), clamped as in walk_exact.
+PICCA_HD inline void walk_key_from(const XiConst &C, int m, int c, double a, double b,
+                                   int &mp, int &cp) {
+    double fm = floor(a) + (double)(m + 1);
+    double fc = floor(b) + (double)(c + 1);
+    fm = fmin(fmax(fm, -C.np_d - 1.0), C.np_d);
+    fc = fmin(fc, C.nt_d);
+    mp = (int)fm;
+    cp = (int)fc;
 }
 
-PICCA_HD inline bool walk_same(WalkKey a, WalkKey b) { return a.m == b.m && a.c == b.c; }
+// Whether a task takes the run walk: delta2 increases strictly with consecutive pixels more
+// than four rounding margins apart in both bin coordinates (spc_r, spc_d: its smallest steps
+// of r_comov and dist_m), and, under the automatic schedule, the window holds at least
+// XI_WALK_MIN_DENSITY pixels per unit of the r_par coordinate (q0).
+PICCA_HD inline bool walk_task(const XiConst &C, double spc_r, double spc_d, double cp, double st,
+                               double q0) {
+    return (C.flags & XI_WALK) && spc_r > 0 && spc_d > 0 && cp * spc_r > 4.0 * (0.5 - C.half_p) &&
+           st * spc_d > 4.0 * (0.5 - C.half_t) &&
+           (!(C.flags & XI_WALK_AUTO) || q0 >= XI_WALK_MIN_DENSITY);
+}
 
 // r_comov2 beyond r_par_max on the far side, or r_trans beyond r_trans_max: every later
 // pixel is rejected too
-PICCA_HD inline bool walk_terminal(const XiConst &C, WalkKey k) {
-    return k.m >= C.np || k.c >= C.nt;
+PICCA_HD inline bool walk_terminal(const XiConst &C, int m, int c) {
+    return m >= C.np || c >= C.nt;
 }
 
 // All pairs of lane pixel L with delta2's window [T.j_lo, T.j_hi).
 //
-// The window is walked as [s, j_hi) then [j_lo, s).  Every lane meets the bins in the same
-// order along j and the lanes of a wave move at about the same pace, so walks that all
-// began at j_lo would add to a few neighbouring bins at a time; the split point s sits at
-// the fraction `phase` of the lane's r_par range [r1 - r_par_max / cos, r1 + r_par_max /
-// cos] (placed with the window's pixel density; any s is exact), so the lanes of a wave
-// add to different bins.
+// Every lane starts at j_lo and walks its runs in order, so the lanes of a wave (consecutive
+// pixels of delta1) probe neighbouring pixels of delta2 at the same time and their loads
+// share cache lines.  (Starting each lane at a different point of its r_par range, so that
+// the lanes add to different bins, and prefetching two runs ahead were both measured
+// slower.)
 //
-// One loop, one memory round trip per pass: every pass loads pixels pa, pb (their r_comov
-// and dist_m) and the prefix entry at pb, evaluates both keys, and then updates the lane's
-// state with short selects, so the lanes of a wave stay on one instruction stream whatever
-// state each is in:
-//   * start of a segment at j (pa = pb = j): the run's key, its prefix entry, the first
+// One loop, one memory round trip per pass: every pass tests pixels pa, pb (their r_comov
+// and dist_m) and the prefix entry at pb against the thresholds of the current run key
+// (Ka = r1 cp + m + 1, Kb = c + 1 - d1 st: a pixel at or after the run's first pixel keeps
+// the key iff r2 cp < Ka and d2 st < Kb):
+//   * start of the walk at j (pa = pb = j): the run's key, its prefix entry, the first
 //     guess of its end;
 //   * test of a candidate end g of the run [j, ...) (pa = g - 1, pb = g): pixel g - 1 with
-//     the run's key and pixel g without it close the run at g; otherwise the bracket
-//     (lo, hi) of the end shrinks (a step back after the first overshoot, then bisection;
-//     galloping forward after an undershoot).
-// A run whose end is guessed right closes in one pass.
+//     the run's key and pixel g without it close the run at g, and pixel g opens the next
+//     run with its own key; otherwise the bracket (lo, hi) of the end shrinks (a step back
+//     after the first overshoot, then bisection; galloping forward after an undershoot).
+// The next pass's loads are issued before the closed run is added to the histogram.  The
+// guess of a run's end is its first pixel plus the distance to the run's upper r_par edge
+// times the pixel density where the previous run closed (the window's mean for the first
+// run), so a run is usually closed in one pass.
+// A coordinate within the rounding margin of an integer is redone with the reference's
+// arithmetic (walk_exact); such a pixel is the first or last pixel of its run and gets the
+// reference's edge, overflow and wrap-around treatment (walk_special).
 template <class Mem>
 PICCA_HD inline void walk_lane(const XiConst &C, const WalkTask &T, const WalkLane &L, Mem &mem) {
-    if (T.j_lo >= T.j_hi) return;
-    const double r_start = L.r1 + (2.0 * L.phase - 1.0) * (C.np_d / T.cp);
-    double s = (double)T.j_lo + (r_start - T.r_lo) * T.cp * T.q0;
-    s = fmin(fmax(s, (double)T.j_lo), (double)T.j_hi);
-    const int split = (int)s;
-    // first segment [split, j_hi); an empty one goes straight to [j_lo, split)
-    bool second = split >= T.j_hi;
-    int seg_end = second ? split : T.j_hi;
-    int j = second ? T.j_lo : split;
-    bool active = j < seg_end;
+    const int j_hi = T.j_hi;
+    if (T.j_lo >= j_hi) return;
+    const int last = j_hi - 1;  // pixel reads stay inside the window
+    const double r1cp = L.r1 * T.cp;
+    const double d1st = L.d1 * T.st;
+    // Before the first run opens the key is (-1, -1), which no pixel has (r_trans >= 0
+    // here), so the thresholds give each pixel's own coordinates.
+    int m = -1, c = -1;
+    double Ka = r1cp, Kb = -d1st;
+    int j = T.j_lo;
     bool starting = true;
-    WalkKey k = {0, 0};
-    double t_j = 0.0, q = T.q0;
-    bool near_j = false, near_lo = false;
     PrefixRecord ps = {0, 0, 0, 0, 0, 0};
-    int lo = j, hi = seg_end, g = j, misses = 0, step = 1;
-    const int last = T.j_hi > 0 ? T.j_hi - 1 : 0;   // pixel reads stay inside the window
-    while (active) {
-        const int pa = starting ? j : g - 1;
-        const int pb = starting ? j : g;
-        double ta, tb;
-        bool na, nb;
-        const WalkKey ka = walk_key(C, T, L, mem, pa, ta, na);
-        const WalkKey kb = walk_key(C, T, L, mem, pb < last ? pb : last, tb, nb);
-        const PrefixRecord pg = mem.prefix(pb);
+    int lo = j, hi = j_hi, g = j, misses = 0, step = 1;
+    bool near_lo = false;
+    // the pass's probes: pixels pa and pb and the prefix entry at pb, loaded at the end of
+    // the previous pass so that the loads overlap that pass's histogram additions
+    int pa = j, pb = j;
+    double ra, da, rb, db;
+    mem.rd(pa, ra, da);
+    mem.rd(pb, rb, db);
+    PrefixRecord pg = mem.prefix(pb);
+    for (;;) {
         mem.count_probe();
-        bool open = false;     // a run starts at pb
-        bool end_seg = false;  // the current segment is done
-        if (starting) {
-            open = true;
-        } else {
-            const bool has1 = g - 1 > lo;
-            const bool has2 = g < seg_end;
-            const bool same1 = !has1 || walk_same(ka, k);
-            const bool same2 = has2 && walk_same(kb, k);
-            if (!same1) {                  // the run ends at g - 1 or before
-                hi = g - 1;
-                g = misses == 0 ? hi : lo + 1 + ((hi - lo) >> 1);
-                ++misses;
-            } else if (same2) {            // the run goes on past g
-                lo = g;
-                near_lo = nb;
-                g = g + step < hi ? g + step : hi;
-                step <<= 1;
-            } else {                       // the run is [j, g)
-                if ((has1 ? na : near_lo) && g - 1 > j) walk_special(C, T, L, mem, g - 1);
-                mem.count_run(pg.c - ps.c);
-                if (k.m >= -C.np) walk_flush(C, T, L, mem, k, ps, pg);
-                if (has2) {
-                    if (g - j >= 3 && tb > t_j) q = (double)(g - j) / (tb - t_j);
-                    open = true;
-                } else {
-                    end_seg = true;
-                }
-            }
+        const double aA = fma(ra, T.cp, -Ka), bA = fma(da, T.st, -Kb);
+        const double aB = fma(rb, T.cp, -Ka), bB = fma(db, T.st, -Kb);
+        // pixel pb: within the margin of any bin edge; pixel pa: of the run's upper edges
+        const bool nearB = (fabs(aB - rint(aB)) < C.fast_p) || (fabs(bB - rint(bB)) < C.fast_t);
+        const bool nearA = (fabs(aA) < C.fast_p) || (fabs(bA) < C.fast_t);
+        int mB, cB;
+        walk_key_from(C, m, c, aB, bB, mB, cB);
+        if (nearB) {
+            const WalkExact e = walk_exact(C, L.r1, L.d1, rb, db, T.cos_half, T.sin_half);
+            mB = e.m;
+            cB = e.c;
         }
-        if (open) {                        // a run starts at pixel pb with key kb
+        bool inA = (aA < 0) && (bA < 0);
+        if (nearA && !starting) {
+            const WalkExact e = walk_exact(C, L.r1, L.d1, ra, da, T.cos_half, T.sin_half);
+            inA = (e.m == m) && (e.c == c);
+        }
+        // outcome of the pass
+        const bool has1 = g - 1 > lo;    // pa is not yet known to have the run's key
+        const bool has2 = g < j_hi;      // pb lies inside the window
+        const bool same1 = !has1 || inA;
+        const bool same2 = has2 && (mB == m) && (cB == c);
+        const bool miss = !starting && !same1;           // the run ends at g - 1 or before
+        const bool gallop = !starting && same1 && same2; // the run goes on past g
+        const bool close = !starting && same1 && !same2; // the run is [j, g)
+        const bool open = starting || (close && has2);   // a run starts at pb
+        const bool terminal = open && walk_terminal(C, mB, cB);
+        const bool done = (close && !has2) || terminal;
+        // the closing run, kept for its histogram addition after the next loads are issued
+        const int m_run = m, c_run = c;
+        const PrefixRecord ps_run = ps;
+        const bool special_last = close && (has1 ? nearA : near_lo) && g - 1 > j;
+        const int p_last = g - 1;
+        const bool special_first = open && nearB;
+        const int p_first = pb;
+        // next state
+        const int g_miss = misses == 0 ? g - 1 : lo + 1 + ((g - 1 - lo) >> 1);
+        const int g_gallop = g + step < hi ? g + step : hi;
+        if (miss) {
+            hi = g - 1;
+            g = g_miss;
+            ++misses;
+        }
+        if (gallop) {
+            lo = g;
+            near_lo = nearB;
+            g = g_gallop;
+            step <<= 1;
+        }
+        if (open && !terminal) {
+            // pixels per unit of the r_par coordinate: from the spacing of pixels g - 1
+            // and g where a run closed here, else the window's mean
+            const double q = starting ? T.q0 : 1.0 / (T.cp * (rb - ra));
             j = pb;
-            k = kb;
-            t_j = tb;
-            near_j = nb;
+            m = mB;
+            c = cB;
             ps = pg;
             starting = false;
-            if (near_j) walk_special(C, T, L, mem, j);
-            if (walk_terminal(C, k)) {
-                end_seg = true;
-            } else {
-                lo = j;
-                hi = seg_end;
-                near_lo = near_j;
-                misses = 0;
-                step = 1;
-                g = walk_guess(k, t_j, q, j, seg_end);
-            }
+            Ka = r1cp + (double)(m + 1);
+            Kb = (double)(c + 1) - d1st;
+            lo = j;
+            hi = j_hi;
+            near_lo = nearB;
+            misses = 0;
+            step = 1;
+            // pixels to the run's upper r_par edge
+            const double ahead = ceil(-fma(rb, T.cp, -Ka) * q);
+            g = j + (int)fmin(fmax(ahead, 1.0), (double)(j_hi - j));
         }
-        if (end_seg) {
-            if (!second && T.j_lo < split) {
-                second = true;
-                seg_end = split;
-                j = T.j_lo;
-                starting = true;
-            } else {
-                active = false;
-            }
+        const PrefixRecord pe = pg;
+        if (!done) {
+            pa = g - 1;
+            pb = g;
+            mem.rd(pa, ra, da);
+            mem.rd(pb < last ? pb : last, rb, db);
+            pg = mem.prefix(pb);
         }
+        if (special_last) walk_special(C, T, L, mem, p_last);
+        if (close) {
+            mem.count_run(pe.c - ps_run.c);
+            if (m_run >= -C.np) walk_flush(C, T, L, mem, m_run, c_run, ps_run, pe);
+        }
+        if (special_first) walk_special(C, T, L, mem, p_first);
+        if (done) break;
     }
 }
 

@@ -275,6 +275,13 @@ class Engine:
         return (weights_dmat, dmat.reshape(nb, nm), r_par_eff, r_trans_eff, z_eff, weight_eff,
                 stats)
 
+    def dmat_report(self):
+        """Kernel times and matrix-core work of the last dmat() call (a dict)."""
+        rep = _native.DmatReport()
+        check(self._lib.picca_dmat_last_report(self._ctx, ctypes.byref(rep)),
+              "picca_dmat_last_report")
+        return rep.as_dict()
+
 
     # -- subsampling covariance --------------------------------------------------
     def cov_subsample(self, xi, weights):

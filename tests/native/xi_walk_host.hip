@@ -205,15 +205,15 @@ int xi_walk_host(const picca_cf_params *params, int64_t num_forests, const int64
                     if (j_d < j_hi) j_hi = j_d;
                 }
             }
+            // an empty window: the kernel returns before choosing a loop
+            if (j_lo >= j_hi) continue;
             const double cp = cos_half * C.inv_p, st = sin_half * C.inv_t;
-            bool walk = (C.flags & picca::XI_WALK) && pruned && spc_r[b] > 0 &&
-                        spc_d[b] > 0 && cp * spc_r[b] > 4.0 * (0.5 - C.half_p) &&
-                        st * spc_d[b] > 4.0 * (0.5 - C.half_t);
             double q0 = 1.0;
             if (j_hi - j_lo >= 2) {
                 const double span = (r[ob + j_hi - 1] - r[ob + j_lo]) * cp;
                 if (span > 0) q0 = (double)(j_hi - 1 - j_lo) / span;
             }
+            const bool walk = pruned && picca::walk_task(C, spc_r[b], spc_d[b], cp, st, q0);
             if (walk) {
                 ++stats[ST_WALK_TASKS];
                 picca::WalkTask T;
@@ -226,7 +226,6 @@ int xi_walk_host(const picca_cf_params *params, int64_t num_forests, const int64
                 T.j_lo = j_lo;
                 T.j_hi = j_hi;
                 T.q0 = q0;
-                T.r_lo = j_hi > j_lo ? r[ob + j_lo] : 0.0;
                 HostMem mem;
                 mem.rp_ = r + ob;
                 mem.dp_ = d + ob;
@@ -240,7 +239,7 @@ int xi_walk_host(const picca_cf_params *params, int64_t num_forests, const int64
                 for (int i = i0; i < i1; ++i) {
                     if (w[oa + i] == 0) continue;
                     const picca::WalkLane L{r[oa + i], d[oa + i], z[oa + i] * 0.5, w[oa + i],
-                                            dw[oa + i], (i - i0 + 0.5) * (1.0 / 64.0)};
+                                            dw[oa + i]};
                     picca::walk_lane(C, T, L, mem);
                 }
             } else {

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     for name in names:
         assert hasattr(lib, name), name
     assert sorted(names) == sorted(_native.exported_symbols())
-    assert lib.picca_abi_version() == _native.ABI_VERSION == 3
+    assert lib.picca_abi_version() == _native.ABI_VERSION == 4
 
 
 def test_library_reports_errors_without_device_work():
