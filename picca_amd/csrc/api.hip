@@ -326,6 +326,7 @@ int picca_xi_set_pairs(picca_ctx *ctx, int64_t num_pairs, const int32_t *pair_a,
     PICCA_HIP_CHECK(ctx->pang.upload(pair_ang, (size_t)num_pairs, ctx->stream));
     PICCA_HIP_CHECK(ctx->pcs.upload(cs.data(), (size_t)(2 * num_pairs), ctx->stream));
     PICCA_HIP_CHECK(ctx->pflags.upload(pair_flags, (size_t)num_pairs, ctx->stream));
+    PICCA_HIP_CHECK(ctx->pchunks.reserve((size_t)std::max<int64_t>(2 * num_pairs, 2)));
     PICCA_HIP_CHECK(ctx->gbegin.upload(ctx->h_group_begin.data(), ctx->h_group_begin.size(),
                                        ctx->stream));
     PICCA_HIP_CHECK(ctx->grow.upload(ctx->h_group_row.data(), ctx->h_group_row.size(),
@@ -390,19 +391,22 @@ int xi_execute_into(picca_ctx *ctx, const picca::XiConst &C0, int mode,
     picca::ForestView F{ctx->pix.ptr, ctx->z.ptr, ctx->r.ptr, ctx->d.ptr, ctx->w.ptr, ctx->dw.ptr,
                         ctx->ll.ptr, ctx->off.ptr, ctx->zq.ptr, ctx->order.ptr, ctx->mono.ptr,
                         ctx->rd.ptr, ctx->pre.ptr, ctx->spc.ptr};
-    picca::PairView P{ctx->pa.ptr, ctx->pb.ptr, ctx->pang.ptr, ctx->pcs.ptr, ctx->pflags.ptr};
+    picca::PairView P{ctx->pa.ptr, ctx->pb.ptr, ctx->pang.ptr, ctx->pcs.ptr, ctx->pflags.ptr,
+                      ctx->pchunks.ptr};
     const int64_t ngroups = (int64_t)ctx->h_group_row.size();
     picca::GroupView G{ctx->gbegin.ptr, ctx->grow.ptr, ngroups};
     picca::XiOut O{out, queue, stats_dev};
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->num_cus, ngroups));
     const bool xcorr = params->x_correlation != 0;
     // the launch sequence: output, queue head and counters cleared by kernels (no memset
-    // nodes in a graph), then the pair kernel
+    // nodes in a graph), the chunk ranges of the pairs, then the pair kernel
     auto enqueue = [&](hipStream_t st) -> hipError_t {
         hipError_t e = picca::launch_zero_u64(reinterpret_cast<unsigned long long *>(out),
                                               nout * (int64_t)(sizeof(picca::XiBin) / 8), st);
         if (e == hipSuccess) e = picca::launch_zero_u64(queue, 1, st);
         if (e == hipSuccess) e = picca::launch_zero_u64(stats_dev, picca::XI_NSTATS, st);
+        if (e == hipSuccess && ngroups > 0)
+            e = picca::launch_xi_chunks(mode, C, F, P, ctx->num_pairs, st);
         if (e == hipSuccess && ngroups > 0)
             e = picca::launch_xi(mode, xcorr, general, lds_hist, C, F, P, G, O, grid, lds, st);
         return e;

@@ -140,7 +140,7 @@ struct picca_ctx {
     // xi pairs
     int64_t num_pairs = 0;
     int32_t num_rows = 0;
-    DeviceBuffer<int32_t> pa, pb;
+    DeviceBuffer<int32_t> pa, pb, pchunks;
     DeviceBuffer<double> pang, pcs;
     DeviceBuffer<uint8_t> pflags;
     std::vector<int64_t> h_group_begin;

@@ -105,6 +105,8 @@ struct PairView {
     const double *__restrict__ ang;
     const double *__restrict__ cs;   // cos(ang/2), sin(ang/2) interleaved
     const uint8_t *__restrict__ flags;
+    int32_t *chunks;                 // per pair: first and end 64-pixel chunk of delta1 that
+                                     // can reach delta2 (xi_chunk_kernel; the xi kernels read)
 };
 
 struct GroupView {
@@ -513,6 +515,9 @@ hipError_t launch_xi(int mode, bool xcorr, bool general, bool lds_hist, const Xi
                      const ForestView &F, const PairView &P, const GroupView &G,
                      const XiOut &O, int grid, size_t lds, hipStream_t stream);
 hipError_t launch_zero_u64(unsigned long long *ptr, int64_t n, hipStream_t stream);
+// per pair, the chunks of delta1 whose pixels can pair with delta2 (P.chunks)
+hipError_t launch_xi_chunks(int mode, const XiConst &C, const ForestView &F, const PairView &P,
+                            int64_t num_pairs, hipStream_t stream);
 // device address of the current GPU's scratch array (XI_SCRATCH_WORDS words)
 hipError_t xi_scratch_address(unsigned long long **ptr);
 hipError_t launch_copy16(const void *src, void *dst, int64_t n16, hipStream_t stream);

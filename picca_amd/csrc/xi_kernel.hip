@@ -575,6 +575,7 @@ xi_kernel(XiConst C, ForestView F, PairView P, GroupView G, XiOut O) {
     __shared__ int64_t s_group;
     __shared__ int s_task;
     __shared__ int s_prefix[XI_GROUP_PAIRS + 1];
+    __shared__ int s_first[XI_GROUP_PAIRS];   // first chunk of each pair of the group
     __shared__ unsigned long long s_stats[XI_NSTATS];
     __shared__ XiConst s_const;
     LocalU64 *lds_stats = (LocalU64 *)s_stats;
@@ -612,9 +613,10 @@ xi_kernel(XiConst C, ForestView F, PairView P, GroupView G, XiOut O) {
         if (tid < 64) {
             int chunks = 0;
             if (tid < npairs) {
-                const int a = P.a[p0 + tid];
-                const int64_t la = F.off[a + 1] - F.off[a];
-                chunks = (int)((la + 63) >> 6);
+                // the chunks of delta1 that can reach delta2 (xi_chunk_kernel)
+                const int first = P.chunks[2 * (p0 + tid)];
+                chunks = P.chunks[2 * (p0 + tid) + 1] - first;
+                s_first[tid] = first;
             }
             int incl = chunks;
             for (int off = 1; off < 64; off <<= 1) {
@@ -643,7 +645,7 @@ xi_kernel(XiConst C, ForestView F, PairView P, GroupView G, XiOut O) {
                 else hi = mid - 1;
             }
             const int q = uniform_int(lo);
-            const int chunk = uniform_int(t - s_prefix[q]);
+            const int chunk = uniform_int(s_first[q] + t - s_prefix[q]);
             const uint64_t steps_before = n_steps;
             if (LDS_HIST) {
                 xi_task<MODE, XCORR, GENERAL, true, WALK>(C, s_const, F, P, p0 + q, chunk, lane, hist,
@@ -694,6 +696,72 @@ hipError_t xi_scratch_address(unsigned long long **ptr) {
     hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
     *ptr = err == hipSuccess ? (unsigned long long *)p : nullptr;
     return err;
+}
+
+// Per pair, the 64-pixel chunks of delta1 [first, end) whose windows of delta2 can be
+// non-empty: with both forests monotone, delta1's pixels must lie within r_par_max / cos of
+// delta2's r_comov span and have dist_m below (r_trans_max / sin) - dist_m of delta2's first
+// pixel, each bound widened by twice the pair kernel's window slack, so every skipped chunk
+// is one whose pruned window the pair kernel would find empty.  Other pairs keep every chunk.
+__global__ void xi_chunk_kernel(XiConst C, ForestView F, PairView P, int64_t num_pairs,
+                                bool standard) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= num_pairs) return;
+    const int a = P.a[p], b = P.b[p];
+    const int64_t oa = F.off[a], ob = F.off[b];
+    const int la = (int)(F.off[a + 1] - oa), lb = (int)(F.off[b + 1] - ob);
+    int first = 0, end = (la + 63) >> 6;
+    const double cos_half = P.cs[2 * p], sin_half = P.cs[2 * p + 1];
+    const int ma = F.mono[a], mb = F.mono[b];
+    if (standard && (C.flags & XI_PRUNE) && cos_half > 0 && (mb & 3) == 3 && (ma & 1) &&
+        la > 0 && lb > 0) {
+        const double *__restrict__ R1 = F.r + oa;
+        const double *__restrict__ D1 = F.d + oa;
+        const double reach = fmax(C.rpmax, fabs(C.rpmin)) / cos_half;
+        const double slack = 8.0 * C.edge_tol / cos_half + 2e-6;
+        double lo = F.r[ob] - reach, hi = F.r[ob + lb - 1] + reach;
+        lo -= slack + 2e-12 * fabs(lo);
+        hi += slack + 2e-12 * fabs(hi);
+        // first pixel with r1 >= lo, first with r1 > hi
+        int i_lo = 0, i_hi = la;
+        for (int n = la; n > 0;) {
+            const int h = n >> 1;
+            if (R1[i_lo + h] < lo) { i_lo += h + 1; n -= h + 1; } else { n = h; }
+        }
+        int base = 0;
+        for (int n = la; n > 0;) {
+            const int h = n >> 1;
+            if (R1[base + h] <= hi) { base += h + 1; n -= h + 1; } else { n = h; }
+        }
+        i_hi = base;
+        if (sin_half > 0 && (ma & 2)) {
+            double d_lim = (C.rtmax + 8.0 * C.edge_tol) / sin_half - F.d[ob];
+            d_lim += 2e-6 + 2e-12 * fabs(d_lim);
+            int k = 0;
+            for (int n = la; n > 0;) {
+                const int h = n >> 1;
+                if (D1[k + h] <= d_lim) { k += h + 1; n -= h + 1; } else { n = h; }
+            }
+            if (k < i_hi) i_hi = k;
+        }
+        if (i_lo >= i_hi) {
+            first = end = 0;
+        } else {
+            first = i_lo >> 6;
+            end = (i_hi + 63) >> 6;
+        }
+    }
+    P.chunks[2 * p] = first;
+    P.chunks[2 * p + 1] = end;
+}
+
+hipError_t launch_xi_chunks(int mode, const XiConst &C, const ForestView &F, const PairView &P,
+                            int64_t num_pairs, hipStream_t stream) {
+    if (num_pairs <= 0) return hipSuccess;
+    const int64_t blocks = (num_pairs + 255) / 256;
+    xi_chunk_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(C, F, P, num_pairs,
+                                                                       mode == MODE_STANDARD);
+    return hipGetLastError();
 }
 
 __global__ void zero_u64_kernel(unsigned long long *__restrict__ a, int64_t n) {

@@ -175,7 +175,25 @@ int xi_walk_host(const picca_cf_params *params, int64_t num_forests, const int64
         Hist &h = rows[(size_t)prow[p]];
         const int64_t oa = off[a], ob = off[b];
         const int la = (int)(off[a + 1] - oa), lb = (int)(off[b + 1] - ob);
-        for (int chunk = 0; chunk * 64 < la; ++chunk) {
+        // the kernel's task list: the chunks of delta1 xi_chunk_kernel keeps
+        int first = 0, end = (la + 63) >> 6;
+        if (cos_half > 0 && (mono[b] & 3) == 3 && (mono[a] & 1) && la > 0 && lb > 0) {
+            const double reach = std::fmax(C.rpmax, std::fabs(C.rpmin)) / cos_half;
+            const double slack = 8.0 * C.edge_tol / cos_half + 2e-6;
+            double lo = r[ob] - reach, hi = r[ob + lb - 1] + reach;
+            lo -= slack + 2e-12 * std::fabs(lo);
+            hi += slack + 2e-12 * std::fabs(hi);
+            const int i_lo = lower(r + oa, la, lo, true);
+            int i_hi = lower(r + oa, la, hi, false);
+            if (sin_half > 0 && (mono[a] & 2)) {
+                double d_lim = (C.rtmax + 8.0 * C.edge_tol) / sin_half - d[ob];
+                d_lim += 2e-6 + 2e-12 * std::fabs(d_lim);
+                i_hi = std::min(i_hi, lower(d + oa, la, d_lim, false));
+            }
+            first = i_lo >= i_hi ? 0 : i_lo >> 6;
+            end = i_lo >= i_hi ? 0 : (i_hi + 63) >> 6;
+        }
+        for (int chunk = first; chunk < end; ++chunk) {
             const int i0 = chunk * 64, i1 = std::min(la, i0 + 64);
             double r_min = INFINITY, r_max = -INFINITY, d_min = INFINITY;
             bool any = false;
