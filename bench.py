@@ -294,8 +294,9 @@ def bench_xi(args):
                 "traffic": args.traffic_bytes if args.traffic_bytes > 0 else None,
                 "flops_per_launch": flops,
                 "executed_tflops": executed / (kernel_ms / 1e3) / 1e12,
+                # the run-walk instance runs whenever any task took the walk
                 "kernel": "picca::xi_kernel<0,false,false,true,%s>"
-                          % ("true" if args.schedule == "runs" else "false"),
+                          % ("true" if rep["walk_tasks"] > 0 else "false"),
                 "kernel_ms": kernel_ms,
                 "hbm_gbs_at_traffic": (args.traffic_bytes / (kernel_ms / 1e3) / 1e9
                                        if args.traffic_bytes > 0 else None)}
