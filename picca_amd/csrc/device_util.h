@@ -73,4 +73,26 @@ __device__ int wave_search(const double *__restrict__ arr, int n, double x, int 
     return lo;
 }
 
+// DPP row moves (gfx9 dpp_ctrl): lane l of a row of 16 reads lane l - n (ROW_SHR) or
+// l + n (ROW_SHL) of the same row, and 0 where that lane lies outside the row (bound_ctrl).
+// Every lane of the wave must be active.
+constexpr int DPP_ROW_SHL1 = 0x101;
+constexpr int DPP_ROW_SHR1 = 0x111;
+constexpr int DPP_ROW_SHR2 = 0x112;
+constexpr int DPP_ROW_SHR4 = 0x114;
+constexpr int DPP_ROW_SHR8 = 0x118;
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_int(int v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true);
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)dpp_int<CTRL>((int)(uint32_t)b);
+    const uint32_t hi = (uint32_t)dpp_int<CTRL>((int)(uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 }  // namespace picca

@@ -165,12 +165,19 @@ PICCA_HD inline void walk_special(const XiConst &C, const WalkTask &T, const Wal
     }
 }
 
-// Adds the run [j, jn) of key (m, c): prefix differences times the lane's pixel.
-template <class Mem>
-PICCA_HD inline void walk_flush(const XiConst &C, const WalkTask &T, const WalkLane &L, Mem &mem,
-                                int m, int c, const PrefixRecord &ps, const PrefixRecord &pe) {
+// Histogram addition of the run [j, jn) of key (m, c): prefix differences times the lane's
+// pixel.  False for an empty run.
+struct RunSums {
+    int bin;
+    double we, xi, rp, rt, z;
+    uint32_t n;
+};
+
+PICCA_HD inline bool walk_run_sums(const XiConst &C, const WalkTask &T, const WalkLane &L, int m,
+                                   int c, const PrefixRecord &ps, const PrefixRecord &pe,
+                                   RunSums &o) {
     const double n = pe.c - ps.c;
-    if (!(n > 0)) return;
+    if (!(n > 0)) return false;
     const int row = m >= 0 ? m : -m - 1;
     const double dw = pe.w - ps.w;
     const double dx = pe.x - ps.x;
@@ -180,10 +187,14 @@ PICCA_HD inline void walk_flush(const XiConst &C, const WalkTask &T, const WalkL
     // sum over the run of w2 (r2 - r1) and w2 (d1 + d2), from the centred sums
     const double s_r = dr - (L.r1 - T.rc) * dw;
     const double s_d = (L.d1 + T.dc) * dw + dd;
-    const double rp = T.cos_half * (L.w1 * (m >= 0 ? s_r : -s_r));
-    const double rt = T.sin_half * (L.w1 * s_d);
-    const double z = L.w1 * (L.zh1 * dw + 0.5 * dz);
-    mem.add(row * C.ntp + c, L.w1 * dw, L.dw1 * dx, rp, rt, z, (uint32_t)n);
+    o.bin = row * C.ntp + c;
+    o.we = L.w1 * dw;
+    o.xi = L.dw1 * dx;
+    o.rp = T.cos_half * (L.w1 * (m >= 0 ? s_r : -s_r));
+    o.rt = T.sin_half * (L.w1 * s_d);
+    o.z = L.w1 * (L.zh1 * dw + 0.5 * dz);
+    o.n = (uint32_t)n;
+    return true;
 }
 
 // Key of pixel p from its coordinates relative to the upper edges of the run key (m, c):
@@ -234,17 +245,20 @@ PICCA_HD inline bool walk_terminal(const XiConst &C, int m, int c) {
 //     the run's key and pixel g without it close the run at g, and pixel g opens the next
 //     run with its own key; otherwise the bracket (lo, hi) of the end shrinks (a step back
 //     after the first overshoot, then bisection; galloping forward after an undershoot).
-// The next pass's loads are issued before the closed run is added to the histogram.  The
-// guess of a run's end is its first pixel plus the distance to the run's upper r_par edge
+// The next pass's loads are issued before the closed run is added to the histogram; the
+// addition is made by all lanes of the wave together (mem.flush), so every lane stays in
+// the loop until the wave's last lane is done (`alive` false: a lane without a pixel, or
+// one past its last run).  The guess of a run's end is its first pixel plus the distance to the run's upper r_par edge
 // times the pixel density where the previous run closed (the window's mean for the first
 // run), so a run is usually closed in one pass.
 // A coordinate within the rounding margin of an integer is redone with the reference's
 // arithmetic (walk_exact); such a pixel is the first or last pixel of its run and gets the
 // reference's edge, overflow and wrap-around treatment (walk_special).
 template <class Mem>
-PICCA_HD inline void walk_lane(const XiConst &C, const WalkTask &T, const WalkLane &L, Mem &mem) {
+PICCA_HD inline void walk_lane(const XiConst &C, const WalkTask &T, const WalkLane &L, Mem &mem,
+                               bool alive) {
     const int j_hi = T.j_hi;
-    if (T.j_lo >= j_hi) return;
+    if (T.j_lo >= j_hi) return;  // wave-uniform
     const int last = j_hi - 1;  // pixel reads stay inside the window
     const double r1cp = L.r1 * T.cp;
     const double d1st = L.d1 * T.st;
@@ -265,91 +279,99 @@ PICCA_HD inline void walk_lane(const XiConst &C, const WalkTask &T, const WalkLa
     mem.rd(pb, rb, db);
     PrefixRecord pg = mem.prefix(pb);
     for (;;) {
-        mem.count_probe();
-        const double aA = fma(ra, T.cp, -Ka), bA = fma(da, T.st, -Kb);
-        const double aB = fma(rb, T.cp, -Ka), bB = fma(db, T.st, -Kb);
-        // pixel pb: within the margin of any bin edge; pixel pa: of the run's upper edges
-        const bool nearB = (fabs(aB - rint(aB)) < C.fast_p) || (fabs(bB - rint(bB)) < C.fast_t);
-        const bool nearA = (fabs(aA) < C.fast_p) || (fabs(bA) < C.fast_t);
-        int mB, cB;
-        walk_key_from(C, m, c, aB, bB, mB, cB);
-        if (nearB) {
-            const WalkExact e = walk_exact(C, L.r1, L.d1, rb, db, T.cos_half, T.sin_half);
-            mB = e.m;
-            cB = e.c;
+        // the run closed in this pass, added by mem.flush with every lane of the wave
+        bool on = false;
+        RunSums run = {0, 0, 0, 0, 0, 0, 0u};
+        if (alive) {
+            mem.count_probe();
+            const double aA = fma(ra, T.cp, -Ka), bA = fma(da, T.st, -Kb);
+            const double aB = fma(rb, T.cp, -Ka), bB = fma(db, T.st, -Kb);
+            // pixel pb: within the margin of any bin edge; pixel pa: of the run's upper edges
+            const bool nearB =
+                (fabs(aB - rint(aB)) < C.fast_p) || (fabs(bB - rint(bB)) < C.fast_t);
+            const bool nearA = (fabs(aA) < C.fast_p) || (fabs(bA) < C.fast_t);
+            int mB, cB;
+            walk_key_from(C, m, c, aB, bB, mB, cB);
+            if (nearB) {
+                const WalkExact e = walk_exact(C, L.r1, L.d1, rb, db, T.cos_half, T.sin_half);
+                mB = e.m;
+                cB = e.c;
+            }
+            bool inA = (aA < 0) && (bA < 0);
+            if (nearA && !starting) {
+                const WalkExact e = walk_exact(C, L.r1, L.d1, ra, da, T.cos_half, T.sin_half);
+                inA = (e.m == m) && (e.c == c);
+            }
+            // outcome of the pass
+            const bool has1 = g - 1 > lo;    // pa is not yet known to have the run's key
+            const bool has2 = g < j_hi;      // pb lies inside the window
+            const bool same1 = !has1 || inA;
+            const bool same2 = has2 && (mB == m) && (cB == c);
+            const bool miss = !starting && !same1;           // the run ends at g - 1 or before
+            const bool gallop = !starting && same1 && same2; // the run goes on past g
+            const bool close = !starting && same1 && !same2; // the run is [j, g)
+            const bool open = starting || (close && has2);   // a run starts at pb
+            const bool terminal = open && walk_terminal(C, mB, cB);
+            const bool done = (close && !has2) || terminal;
+            const bool special_last = close && (has1 ? nearA : near_lo) && g - 1 > j;
+            const int p_last = g - 1;
+            const bool special_first = open && nearB;
+            const int p_first = pb;
+            // the closing run [j, g): prefix entries ps and pg (the entry at g)
+            if (close) {
+                mem.count_run(pg.c - ps.c);
+                if (m >= -C.np) on = walk_run_sums(C, T, L, m, c, ps, pg, run);
+            }
+            // next state
+            const int g_miss = misses == 0 ? g - 1 : lo + 1 + ((g - 1 - lo) >> 1);
+            const int g_gallop = g + step < hi ? g + step : hi;
+            if (miss) {
+                hi = g - 1;
+                g = g_miss;
+                ++misses;
+            }
+            if (gallop) {
+                lo = g;
+                near_lo = nearB;
+                g = g_gallop;
+                step <<= 1;
+            }
+            if (open && !terminal) {
+                // pixels per unit of the r_par coordinate: from the spacing of pixels g - 1
+                // and g where a run closed here, else the window's mean
+                const double q = starting ? T.q0 : 1.0 / (T.cp * (rb - ra));
+                j = pb;
+                m = mB;
+                c = cB;
+                ps = pg;
+                starting = false;
+                Ka = r1cp + (double)(m + 1);
+                Kb = (double)(c + 1) - d1st;
+                lo = j;
+                hi = j_hi;
+                near_lo = nearB;
+                misses = 0;
+                step = 1;
+                // pixels to the run's upper r_par edge
+                const double ahead = ceil(-fma(rb, T.cp, -Ka) * q);
+                g = j + (int)fmin(fmax(ahead, 1.0), (double)(j_hi - j));
+            }
+            if (!done) {
+                pa = g - 1;
+                pb = g;
+            }
+            if (special_last) walk_special(C, T, L, mem, p_last);
+            if (special_first) walk_special(C, T, L, mem, p_first);
+            if (done) alive = false;
         }
-        bool inA = (aA < 0) && (bA < 0);
-        if (nearA && !starting) {
-            const WalkExact e = walk_exact(C, L.r1, L.d1, ra, da, T.cos_half, T.sin_half);
-            inA = (e.m == m) && (e.c == c);
-        }
-        // outcome of the pass
-        const bool has1 = g - 1 > lo;    // pa is not yet known to have the run's key
-        const bool has2 = g < j_hi;      // pb lies inside the window
-        const bool same1 = !has1 || inA;
-        const bool same2 = has2 && (mB == m) && (cB == c);
-        const bool miss = !starting && !same1;           // the run ends at g - 1 or before
-        const bool gallop = !starting && same1 && same2; // the run goes on past g
-        const bool close = !starting && same1 && !same2; // the run is [j, g)
-        const bool open = starting || (close && has2);   // a run starts at pb
-        const bool terminal = open && walk_terminal(C, mB, cB);
-        const bool done = (close && !has2) || terminal;
-        // the closing run, kept for its histogram addition after the next loads are issued
-        const int m_run = m, c_run = c;
-        const PrefixRecord ps_run = ps;
-        const bool special_last = close && (has1 ? nearA : near_lo) && g - 1 > j;
-        const int p_last = g - 1;
-        const bool special_first = open && nearB;
-        const int p_first = pb;
-        // next state
-        const int g_miss = misses == 0 ? g - 1 : lo + 1 + ((g - 1 - lo) >> 1);
-        const int g_gallop = g + step < hi ? g + step : hi;
-        if (miss) {
-            hi = g - 1;
-            g = g_miss;
-            ++misses;
-        }
-        if (gallop) {
-            lo = g;
-            near_lo = nearB;
-            g = g_gallop;
-            step <<= 1;
-        }
-        if (open && !terminal) {
-            // pixels per unit of the r_par coordinate: from the spacing of pixels g - 1
-            // and g where a run closed here, else the window's mean
-            const double q = starting ? T.q0 : 1.0 / (T.cp * (rb - ra));
-            j = pb;
-            m = mB;
-            c = cB;
-            ps = pg;
-            starting = false;
-            Ka = r1cp + (double)(m + 1);
-            Kb = (double)(c + 1) - d1st;
-            lo = j;
-            hi = j_hi;
-            near_lo = nearB;
-            misses = 0;
-            step = 1;
-            // pixels to the run's upper r_par edge
-            const double ahead = ceil(-fma(rb, T.cp, -Ka) * q);
-            g = j + (int)fmin(fmax(ahead, 1.0), (double)(j_hi - j));
-        }
-        const PrefixRecord pe = pg;
-        if (!done) {
-            pa = g - 1;
-            pb = g;
-            mem.rd(pa, ra, da);
-            mem.rd(pb < last ? pb : last, rb, db);
-            pg = mem.prefix(pb);
-        }
-        if (special_last) walk_special(C, T, L, mem, p_last);
-        if (close) {
-            mem.count_run(pe.c - ps_run.c);
-            if (m_run >= -C.np) walk_flush(C, T, L, mem, m_run, c_run, ps_run, pe);
-        }
-        if (special_first) walk_special(C, T, L, mem, p_first);
-        if (done) break;
+        // the next pass's probes, issued before the histogram additions (a lane that is done
+        // reloads its last probes, which lie inside the window)
+        const bool more = mem.any(alive);
+        mem.rd(pa, ra, da);
+        mem.rd(pb < last ? pb : last, rb, db);
+        pg = mem.prefix(pb);
+        mem.flush(on, run);
+        if (!more) break;
     }
 }
 

@@ -71,6 +71,10 @@ struct HostMem {
         hist->z[bin] += zz;
         hist->nb[bin] += n;
     }
+    void flush(bool on, const picca::RunSums &r) {
+        if (on) add(r.bin, r.we, r.xi, r.rp, r.rt, r.z, r.n);
+    }
+    bool any(bool v) { return v; }
     void count_edge() { ++stats[ST_EDGE]; }
     void count_overflow() { ++stats[ST_OVER]; }
     void count_probe() { ++stats[ST_PROBES]; }
@@ -258,7 +262,7 @@ int xi_walk_host(const picca_cf_params *params, int64_t num_forests, const int64
                     if (w[oa + i] == 0) continue;
                     const picca::WalkLane L{r[oa + i], d[oa + i], z[oa + i] * 0.5, w[oa + i],
                                             dw[oa + i]};
-                    picca::walk_lane(C, T, L, mem);
+                    picca::walk_lane(C, T, L, mem, true);
                 }
             } else {
                 ++stats[ST_PAIR_TASKS];
