@@ -183,7 +183,7 @@ def main():
     parser.add_argument("--cpu-pairs", type=float, default=1.2e10,
                         help="evaluated pixel pairs in the CPU-baseline sample")
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    parser.add_argument("--time-budget", type=float, default=420.0,
+    parser.add_argument("--time-budget", type=float, default=120.0,
                         help="wall seconds for the warm-up and timed GPU steps together; "
                              "the steps that fit are run and reported")
     parser.add_argument("--copy-probe", type=float, default=0.0,
