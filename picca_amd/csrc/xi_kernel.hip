@@ -211,8 +211,7 @@ struct DeviceWalkMem {
     const double *__restrict__ dwp;
     LocalBin *hist;
     LocalU64 *stats;
-    uint32_t probes, runs;
-    double cover;
+    uint32_t probes, runs, cover;
 
     __device__ void rd(int p, double &r, double &d) const {
         const double2 v = rdp[p];
@@ -273,6 +272,13 @@ struct DeviceWalkMem {
         head |= h;
     }
     __device__ bool any(bool v) const { return __ballot(v) != 0; }
+    // Keeps the run's sums where the walk computes them, at the top of the pass: left to
+    // itself the compiler moves them to the flush, after the prefix record of the run's
+    // start has been replaced, and keeps two copies of that record around the pass.
+    __device__ static void pin(RunSums &r) {
+        asm volatile("" : "+v"(r.we), "+v"(r.xi), "+v"(r.rp), "+v"(r.rt), "+v"(r.z), "+v"(r.n),
+                     "+v"(r.bin));
+    }
     __device__ void count_edge() {
         __hip_atomic_fetch_add(&stats[XI_STAT_EDGE], 1ull, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -282,7 +288,7 @@ struct DeviceWalkMem {
                                __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     __device__ void count_probe() { ++probes; }
-    __device__ void count_run(double n) {
+    __device__ void count_run(uint32_t n) {
         ++runs;
         cover += n;
     }
@@ -401,10 +407,10 @@ __device__ void xi_task(const XiConst &C, const XiConst &C_lds, const ForestView
                 T.j_hi = j_hi;
                 T.q0 = q0;
                 DeviceWalkMem mem{F.rd + ob, F.pre + (ob + b), F.w + ob, F.z + ob, F.dw + ob,
-                                  hist, lds_stats, 0u, 0u, 0.0};
+                                  hist, lds_stats, 0u, 0u, 0u};
                 const WalkLane L{r1, d1, zh1, w1, dw1};
                 walk_lane(C, T, L, mem, active);
-                cnt.visit += (uint64_t)mem.cover;
+                cnt.visit += mem.cover;
                 cnt.runs += mem.runs;
                 cnt.probes += mem.probes;
                 ++n_walk;

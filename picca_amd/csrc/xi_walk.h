@@ -292,6 +292,7 @@ PICCA_HD inline void walk_lane(const XiConst &C, const WalkTask &T, const WalkLa
         // the values the flush moves between lanes are finite
         RunSums run;
         const bool nonempty = walk_run_sums(C, T, L, m, c, ps, pg, run);
+        mem.pin(run);
         bool on = false;
         if (alive) {
             mem.count_probe();
@@ -330,7 +331,7 @@ PICCA_HD inline void walk_lane(const XiConst &C, const WalkTask &T, const WalkLa
             const int p_first = pb;
             // the closing run [j, g): prefix entries ps and pg (the entry at g)
             if (close) {
-                mem.count_run(pg.c - ps.c);
+                mem.count_run(run.n);
                 on = nonempty && m >= -C.np;
             }
             // next state

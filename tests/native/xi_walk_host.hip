@@ -75,10 +75,11 @@ struct HostMem {
         if (on) add(r.bin, r.we, r.xi, r.rp, r.rt, r.z, r.n);
     }
     bool any(bool v) { return v; }
+    void pin(picca::RunSums &) {}
     void count_edge() { ++stats[ST_EDGE]; }
     void count_overflow() { ++stats[ST_OVER]; }
     void count_probe() { ++stats[ST_PROBES]; }
-    void count_run(double n) {
+    void count_run(uint32_t n) {
         ++stats[ST_RUNS];
         stats[ST_COVER] += (int64_t)n;
     }
