@@ -196,8 +196,8 @@ PICCA_HD inline bool walk_run_sums(const XiConst &C, const WalkTask &T, const Wa
 }
 
 // Reciprocal of x > 0 to about 3e-4 relative, the same on the host and the device: the
-// seed 2^(2 * 1023) / x read off the bit pattern (within 12.5 %), then two Newton steps
-// y (2 - x y).  For the guess of a run's end only.
+// seed is the bit pattern of 2.0^1023 (0x7FE0...) minus that of x, within 12.5 % of 1 / x
+// for a normal x, then two Newton steps y (2 - x y).  For the guess of a run's end only.
 PICCA_HD inline double walk_rcp_guess(double x) {
     const uint64_t b = 0x7FE0000000000000ull - __builtin_bit_cast(uint64_t, x);
     double y = __builtin_bit_cast(double, b);
@@ -256,9 +256,9 @@ PICCA_HD inline bool walk_terminal(const XiConst &C, int m, int c) {
 // The next pass's loads are issued before the closed run is added to the histogram; the
 // addition is made by all lanes of the wave together (mem.flush), so every lane stays in
 // the loop until the wave's last lane is done (`alive` false: a lane without a pixel, or
-// one past its last run).  The guess of a run's end is its first pixel plus the distance to the run's upper r_par edge
-// times the pixel density where the previous run closed (the window's mean for the first
-// run), so a run is usually closed in one pass.
+// one past its last run).  The guess of a run's end is its first pixel plus the distance
+// to the run's upper r_par edge times the pixel density where the previous run closed (the
+// window's mean for the first run), so a run is usually closed in one pass.
 // A coordinate within the rounding margin of an integer is redone with the reference's
 // arithmetic (walk_exact); such a pixel is the first or last pixel of its run and gets the
 // reference's edge, overflow and wrap-around treatment (walk_special).
