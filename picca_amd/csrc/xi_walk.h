@@ -195,7 +195,7 @@ PICCA_HD inline bool walk_run_sums(const XiConst &C, const WalkTask &T, const Wa
     return n > 0;
 }
 
-// Reciprocal of x > 0 to about 3e-4 relative, the same on the host and the device: the
+// Reciprocal of x > 0 within 2.5e-4 relative, the same on the host and the device: the
 // seed is the bit pattern of 2.0^1023 (0x7FE0...) minus that of x, within 12.5 % of 1 / x
 // for a normal x, then two Newton steps y (2 - x y).  For the guess of a run's end only.
 PICCA_HD inline double walk_rcp_guess(double x) {

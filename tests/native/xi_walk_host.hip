@@ -133,6 +133,9 @@ void exact_pair(const XiConst &C, double r1, double d1, double zh1, double w1, d
 
 extern "C" {
 
+// The walk's guess reciprocal (walk_rcp_guess), for its unit test.
+double xi_walk_rcp_guess(double x) { return picca::walk_rcp_guess(x); }
+
 // Correlation of the listed forest pairs (specialised case of the pair kernel only).
 // Outputs are [num_rows * np * nt] each; stats = edge, overflow, walked tasks, pair-by-pair
 // tasks, runs, probes, out-of-bounds accesses, pixel pairs covered by runs.  Returns 0, or

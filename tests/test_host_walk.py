@@ -75,3 +75,18 @@ def test_zero_weights_short_and_irregular_forests():
 @pytest.mark.parametrize("seed", range(12))
 def test_random_geometries(seed):
     run(W.random_case(seed), rtol=1e-8)
+
+
+def test_reciprocal_guess_of_the_run_end():
+    """walk_rcp_guess: the seed (bit pattern of 2^1023 minus that of x) is within 12.5 % of
+    1 / x, and each Newton step squares the relative error: (1/8)^4 = 2.4e-4 at most.  The
+    worst case is the mantissa 1.5 (seed 0.75 / 2^k, x * seed = 9/8); powers of two are
+    exact."""
+    import numpy as np
+    rng = np.random.default_rng(7)
+    xs = np.concatenate([10.0 ** rng.uniform(-12, 12, 4000), 2.0 ** np.arange(-40, 41),
+                         1.5 * 2.0 ** np.arange(-40, 41), [0.05, 0.25, 4.0 / 3.0]])
+    err = np.array([abs(x * H.rcp_guess(x) - 1.0) for x in xs])
+    assert err.max() < 2.5e-4
+    assert err[4000:4081].max() == 0.0                     # powers of two
+    np.testing.assert_allclose(err[4081:4162], 0.125 ** 4, rtol=1e-9)

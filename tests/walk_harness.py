@@ -43,7 +43,14 @@ def lib():
             build()
         _lib = ctypes.CDLL(LIBRARY)
         _lib.xi_walk_host.restype = ctypes.c_int
+        _lib.xi_walk_rcp_guess.restype = ctypes.c_double
+        _lib.xi_walk_rcp_guess.argtypes = [ctypes.c_double]
     return _lib
+
+
+def rcp_guess(x):
+    """The walk's reciprocal guess of x > 0 (the same code the kernel runs)."""
+    return lib().xi_walk_rcp_guess(float(x))
 
 
 def _p(a):
